@@ -1,0 +1,230 @@
+"""MI355X-native BS(n+1,n) first-hit engine: Python binding of libwzfh.so (C ABI in include/wzfh.h).
+
+The library is the product: HIP kernels for the C,D cell stream (with the FP64 hall_ok
+leaf test) and the A,B completer, plus the per-(n, class) host setup. This module is a
+thin ctypes layer with the reference's vocabulary (cells, candidates, completions). It
+has no CPU compute path: if libwzfh.so is missing or no GPU is present, calls raise.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIBPATH = PKG / 'libwzfh.so'
+CLIPATH = PKG / 'wzfh_match'
+
+OK, EINVAL, ENOMEM, EDEVICE, ENODEV, ECAPACITY, EINTERNAL, ETRUNC = 0, -1, -2, -3, -4, -5, -6, -7
+HIT, PREREJECT, CLEAN_NO, ABORTED, INTERNAL = 0, 1, 2, 3, 4
+
+EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy', 'wzfh_last_error',
+           'wzfh_num_cells', 'wzfh_cell_info', 'wzfh_order_digest', 'wzfh_stream_cells', 'wzfh_complete',
+           'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms')
+
+
+class WzfhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f'wzfh error {code}: {msg}')
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ('n', 'a', 'b', 'c', 'd', 'm6', 'thm211b', 'thm212', 'canon', 'orbit_q',
+                                        'endpos', 'order', 'rev', 'ab_prof', 'outer_k', 'no_canon_ab')] + \
+               [('abp_cap', C.c_longlong)]
+
+
+class StreamOpts(C.Structure):
+    _fields_ = [('cand_cap', C.c_longlong), ('node_budget', C.c_longlong), ('split_depth', C.c_int),
+                ('window', C.c_int)]
+
+
+class CellStats(C.Structure):
+    _fields_ = [('cand', C.c_longlong), ('leaves', C.c_longlong), ('hall_ok', C.c_longlong),
+                ('dfs_nodes', C.c_longlong), ('capped', C.c_int), ('reserved', C.c_int)]
+
+
+class Completion(C.Structure):
+    _fields_ = [('verdict', C.c_int), ('reserved', C.c_int), ('nodes', C.c_longlong), ('A', C.c_uint64),
+                ('B', C.c_uint64)]
+
+
+class SearchOpts(C.Structure):
+    _fields_ = [('stream', StreamOpts), ('top_k', C.c_int), ('batched', C.c_int), ('budget', C.c_longlong)]
+
+
+class SearchResult(C.Structure):
+    _fields_ = [('stream', CellStats), ('completed', C.c_int), ('hit', C.c_int), ('hit_rank', C.c_longlong),
+                ('hit_idx', C.c_longlong), ('ab_nodes', C.c_longlong), ('clean_no', C.c_int), ('aborted', C.c_int),
+                ('A', C.c_uint64), ('B', C.c_uint64), ('C', C.c_uint64), ('D', C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libwzfh.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIBPATH.exists():
+            raise FileNotFoundError(f'{LIBPATH} not built: run python -m bs45_quantum_explorer_amd.build')
+        L = C.CDLL(str(LIBPATH))
+        vp = C.c_void_p
+        L.wzfh_config_default.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.wzfh_init.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.wzfh_init_host.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.wzfh_destroy.argtypes = [vp]
+        L.wzfh_destroy.restype = None
+        L.wzfh_last_error.restype = C.c_char_p
+        L.wzfh_num_cells.argtypes = [vp]
+        L.wzfh_cell_info.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.wzfh_order_digest.argtypes = [vp, C.c_char_p, C.c_int]
+        L.wzfh_stream_cells.argtypes = [vp, vp, C.c_int, C.POINTER(StreamOpts), vp, vp, C.c_longlong, vp]
+        L.wzfh_complete.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
+        L.wzfh_search_cells.argtypes = [vp, vp, C.c_int, C.POINTER(SearchOpts), vp]
+        L.wzfh_npaf_check.argtypes = [C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int)]
+        L.wzfh_flat_score.argtypes = [C.c_int, C.c_uint64, C.c_uint64]
+        L.wzfh_flat_score.restype = C.c_longlong
+        L.wzfh_last_kernel_ms.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise WzfhError(rc, lib().wzfh_last_error().decode())
+    return rc
+
+
+def pack(seq):
+    """+1/-1 sequence -> packed word (bit i set when element i is -1)."""
+    w = 0
+    for i, v in enumerate(seq):
+        if int(v) < 0:
+            w |= 1 << i
+    return w
+
+
+def unpack(word, length):
+    return np.array([-1 if (int(word) >> i) & 1 else 1 for i in range(length)], dtype=np.int8)
+
+
+def config(n, sig, **flags):
+    cfg = Config()
+    _check(lib().wzfh_config_default(C.byref(cfg), n, *sig))
+    for k, v in flags.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Problem:
+    """One (n, signature class) with its ordered cell list, rows and device tables."""
+
+    def __init__(self, n, sig, device=True, **flags):
+        self.n = n
+        self.sig = tuple(sig)
+        self.cfg = config(n, sig, **flags)
+        self.m = 6 if self.cfg.m6 else 3
+        h = C.c_void_p()
+        fn = lib().wzfh_init if device else lib().wzfh_init_host
+        _check(fn(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().wzfh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def num_cells(self):
+        return lib().wzfh_num_cells(self.h)
+
+    def cell(self, pi):
+        px = (C.c_int * 8)()
+        py = (C.c_int * 8)()
+        kept, rows = C.c_int(), C.c_int()
+        _check(lib().wzfh_cell_info(self.h, pi, px, py, C.byref(kept), C.byref(rows)))
+        return list(px[:self.m]), list(py[:self.m]), kept.value, rows.value
+
+    def live_cells(self):
+        """Cells the arm loop streams: kept by orbit canon and not proven dead."""
+        out = []
+        for pi in range(self.num_cells()):
+            _, _, kept, rows = self.cell(pi)
+            if kept and rows != -1:
+                out.append(pi)
+        return out
+
+    def digest(self):
+        buf = C.create_string_buffer(64)
+        _check(lib().wzfh_order_digest(self.h, buf, 64))
+        return buf.value.decode()
+
+    def stream(self, cells, cand_cap=0, node_budget=0, split_depth=0, window=0, want_cands=True):
+        cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
+        nc = len(cells)
+        opts = StreamOpts(cand_cap, node_budget, split_depth, window)
+        stats = (CellStats * nc)()
+        offs = np.zeros(nc + 1, dtype=np.int64)
+        _check(lib().wzfh_stream_cells(self.h, cells.ctypes.data, nc, C.byref(opts), stats, None, 0,
+                                       offs.ctypes.data))
+        cd = None
+        if want_cands:
+            total = int(offs[-1])
+            cd = np.zeros((max(total, 1), 2), dtype=np.uint64)
+            _check(lib().wzfh_stream_cells(self.h, cells.ctypes.data, nc, C.byref(opts), stats, cd.ctypes.data,
+                                           total, offs.ctypes.data))
+            cd = cd[:total]
+        return stats, offs, cd
+
+    def complete(self, cd, budget=0, cell_of=None):
+        cd = np.ascontiguousarray(np.asarray(cd, dtype=np.uint64).reshape(-1, 2))
+        nc = len(cd)
+        out = (Completion * max(nc, 1))()
+        co = None
+        if cell_of is not None:
+            co = np.ascontiguousarray(np.asarray(cell_of, dtype=np.int32))
+        rc = lib().wzfh_complete(self.h, cd.ctypes.data, co.ctypes.data if co is not None else None, nc,
+                                 budget, out)
+        if rc not in (OK, EINTERNAL):
+            _check(rc)
+        return [out[i] for i in range(nc)]
+
+    def search(self, cells, top_k, budget, cand_cap=0, node_budget=0, batched=True, split_depth=0, window=0):
+        cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
+        nc = len(cells)
+        opts = SearchOpts(StreamOpts(cand_cap, node_budget, split_depth, window), top_k, int(batched), budget)
+        res = (SearchResult * nc)()
+        rc = lib().wzfh_search_cells(self.h, cells.ctypes.data, nc, C.byref(opts), res)
+        if rc not in (OK, EINTERNAL):
+            _check(rc)
+        return [res[i] for i in range(nc)]
+
+    def last_kernel(self, kind):
+        ms, units = C.c_double(), C.c_longlong()
+        _check(lib().wzfh_last_kernel_ms(self.h, kind, C.byref(ms), C.byref(units)))
+        return ms.value, units.value
+
+
+def flat_score(n, Cw, Dw):
+    return lib().wzfh_flat_score(n, int(Cw), int(Dw))
+
+
+def npaf_max(n, A, B, Cs, Ds):
+    arrs = [np.ascontiguousarray(np.asarray(x, dtype=np.int8)) for x in (A, B, Cs, Ds)]
+    mx = C.c_int()
+    _check(lib().wzfh_npaf_check(n, *[a.ctypes.data for a in arrs], C.byref(mx)))
+    return mx.value
+
+
+def dump_lines(n, cd):
+    """WZ_FH_DUMP text (wz_match.cpp:2743-2748) of packed candidates."""
+    out = []
+    for X, Y in cd:
+        X, Y = int(X), int(Y)
+        c = ''.join(f'{-1 if (X >> i) & 1 else 1} ' for i in range(n))
+        d = ''.join(f'{-1 if (Y >> i) & 1 else 1} ' for i in range(n))
+        out.append(c + '| ' + d + '\n')
+    return out

@@ -1,0 +1,674 @@
+// wzfh_api.hip -- host runtime and C ABI of libwzfh.so.
+//
+// Owns the device copies of a problem (cells, rows, hall tables, A1 tables), the
+// per-call work buffers and the HIP stream, launches the stream and completer
+// kernels, and turns per-item counts into the reference's exact global order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "wzfh_internal.h"
+#include "wzfh_kernels.h"
+
+namespace wzfh {
+
+hipError_t launch_stream(const StreamArgs &a, int blocks, hipStream_t st);
+hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, hipStream_t st);
+
+static thread_local std::string g_err;
+std::string set_error(const std::string &msg) { g_err = msg; return msg; }
+
+#define WZ_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                    \
+      return WZFH_EDEVICE;                                                             \
+    }                                                                                  \
+  } while (0)
+
+// Growable device buffer.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return WZFH_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t b = std::max<size_t>(want, 256);
+    if (hipMalloc(&p, b) != hipSuccess) { p = nullptr; set_error("hipMalloc failed"); return WZFH_ENOMEM; }
+    bytes = b;
+    return WZFH_OK;
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+struct DeviceState {
+  int device = 0, num_cu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // problem constants
+  DBuf hcos, hsin, order, rej, outer2, outer3, posidx, rows, kab;
+  StreamArgs base{};                       // tables + constants of the stream kernel
+  std::vector<uint8_t> angle_order;        // current evaluation order (angle j-1)
+  std::vector<unsigned long long> angle_hist;
+  std::vector<uint8_t> kab_host;
+  // per-call buffers
+  DBuf cell_res, cell_rootok, cut, item_nodes, item_leaves, item_cands, cands, ctr;
+  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_scratch;
+  double last_ms[2] = {0, 0};
+  long long last_units[2] = {0, 0};
+};
+
+static int quad_bits(const int *v) {   // sign bits x1,y1,x2,y2 -> bit3..bit0 (set = -1)
+  return ((v[0] < 0) << 3) | ((v[1] < 0) << 2) | ((v[2] < 0) << 1) | (v[3] < 0);
+}
+
+static int class_count(int L, int c, int m) { return c < L ? (L - c + m - 1) / m : 0; }
+
+// Stream-kernel constants of a problem: child tables in DFS digit order (with
+// WZ_FH_STREAM_REV and the C,D root pins folded in), per-depth class info.
+static void build_stream_tables(const wzfh_problem &P, StreamArgs &a) {
+  const int L = P.n, m = P.m, half = L / 2;
+  const bool rev = P.cfg.rev != 0, endpos = P.cfg.endpos != 0;
+  a.L = L; a.half = half; a.m = m;
+  a.odd = L & 1;
+  a.cm_mid = half % m;
+  a.nch0 = endpos ? 8 : 16;
+  int tot[8];
+  for (int c = 0; c < m; c++) tot[c] = class_count(L, c, m);
+  for (int d = 0; d < 32; d++) {
+    if (d >= half) { a.depth_info[d] = 0; continue; }
+    const int c1 = d % m, c2 = (L - 1 - d) % m;
+    auto rem_after = [&](int c) {   // remaining positions of class c once depths 0..d are placed
+      int placed = 0;
+      for (int i = c; i < L; i += m) if (i <= d || i >= L - 1 - d) placed++;
+      return tot[c] - placed;
+    };
+    a.depth_info[d] = (uint32_t)c1 | ((uint32_t)c2 << 8) | ((uint32_t)rem_after(c1) << 16) | ((uint32_t)rem_after(c2) << 24);
+  }
+  auto quad_of = [&](int d, int b) -> const int * {
+    if (d == 0) {
+      const int qi = rev ? a.nch0 - 1 - b : b;
+      return endpos ? kQuads.pos[qi] : kQuads.all[qi];
+    }
+    return kQuads.pos[rev ? 7 - b : b];
+  };
+  for (int b = 0; b < 8; b++) a.quad8[b] = (uint8_t)quad_bits(quad_of(1, b));
+  a.root_pin_mask = 0;
+  for (int b = 0; b < 16; b++) {
+    if (b >= a.nch0) { a.quad16[b] = 0; a.root_pin_mask |= 1 << b; continue; }
+    const int *v = quad_of(0, b);
+    a.quad16[b] = (uint8_t)quad_bits(v);
+    if ((P.pinC && v[0] != 1) || (P.pinD && v[1] != 1)) a.root_pin_mask |= 1 << b;
+  }
+  auto pm_bit = [](int v) { return v > 0 ? 0 : 1; };
+  auto sum_bit = [](int s) { return s == 2 ? 0 : (s == 0 ? 1 : 2); };
+  for (int idx = 0; idx < 256; idx++) {
+    const int mx1 = idx & 3, my1 = (idx >> 2) & 3, mx2 = (idx >> 4) & 3, my2 = (idx >> 6) & 3;
+    unsigned m8 = 0, m16 = 0;
+    for (int b = 0; b < 16; b++) {
+      for (int depth = 0; depth < 2; depth++) {
+        if (depth == 1 && b >= 8) continue;
+        if (depth == 0 && b >= a.nch0) continue;
+        const int *v = quad_of(depth == 0 ? 0 : 1, b);
+        bool ok = ((mx1 >> pm_bit(v[0])) & 1) && ((my1 >> pm_bit(v[1])) & 1) && ((mx2 >> pm_bit(v[2])) & 1) &&
+                  ((my2 >> pm_bit(v[3])) & 1);
+        if (!ok) continue;
+        if (depth == 1) m8 |= 1u << b;
+        else if (!((a.root_pin_mask >> b) & 1)) m16 |= 1u << b;
+      }
+    }
+    a.tab_ne[idx] = (uint8_t)m8;
+    a.tab0_ne[idx] = (uint16_t)m16;
+  }
+  for (int idx = 0; idx < 64; idx++) {
+    const int sx = idx & 7, sy = (idx >> 3) & 7;
+    unsigned m8 = 0, m16 = 0;
+    for (int b = 0; b < 16; b++) {
+      for (int depth = 0; depth < 2; depth++) {
+        if (depth == 1 && b >= 8) continue;
+        if (depth == 0 && b >= a.nch0) continue;
+        const int *v = quad_of(depth == 0 ? 0 : 1, b);
+        bool ok = ((sx >> sum_bit(v[0] + v[2])) & 1) && ((sy >> sum_bit(v[1] + v[3])) & 1);
+        if (!ok) continue;
+        if (depth == 1) m8 |= 1u << b;
+        else if (!((a.root_pin_mask >> b) & 1)) m16 |= 1u << b;
+      }
+    }
+    a.tab_eq[idx] = (uint8_t)m8;
+    a.tab0_eq[idx] = (uint16_t)m16;
+  }
+  a.lim = 4.0 * P.n + 2.0 + 0.5;
+}
+
+// Kab(D, s): 2 * number of shift-s pairs of an A,B of length L not yet fully placed
+// when D mirror quads are placed (the reference's incremental Kab, wz_match.cpp:1313-1314,
+// 1115-1138, in closed form).
+static std::vector<uint8_t> build_kab(int L) {
+  std::vector<uint8_t> t(32 * 64, 0);
+  for (int D = 0; D < 32; D++)
+    for (int s = 1; s < 64 && s < L; s++) {
+      int both = 0;
+      for (int i = 0; i + s < L; i++) {
+        const bool pi = i < D || i >= L - D, pj = (i + s) < D || (i + s) >= L - D;
+        if (pi && pj) both++;
+      }
+      t[D * 64 + s] = (uint8_t)(2 * (L - s - both));
+    }
+  return t;
+}
+
+int device_init(wzfh_problem &P) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("no HIP device: libwzfh has no CPU path");
+    return WZFH_ENODEV;
+  }
+  DeviceState *S = new DeviceState();
+  P.dev = S;
+  WZ_HIP(hipGetDevice(&S->device));
+  hipDeviceProp_t prop;
+  WZ_HIP(hipGetDeviceProperties(&prop, S->device));
+  S->num_cu = prop.multiProcessorCount;
+  WZ_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  for (auto &e : S->ev) WZ_HIP(hipEventCreate(&e));
+  const size_t tb = 201 * 64 * sizeof(double);
+  if (S->hcos.ensure(tb) || S->hsin.ensure(tb)) return WZFH_ENOMEM;
+  WZ_HIP(hipMemcpy(S->hcos.p, P.hcos, tb, hipMemcpyHostToDevice));
+  WZ_HIP(hipMemcpy(S->hsin.p, P.hsin, tb, hipMemcpyHostToDevice));
+  // initial angle order: j = 1..100 spread (golden-ratio stride), then 101..200; the
+  // order is refined from the first-rejection histogram after every launch
+  S->angle_order.clear();
+  std::vector<int> seen(201, 0);
+  for (int t = 0, j = 0; t < 100; t++) {
+    j = (j + 61) % 100;
+    while (seen[j + 1]) j = (j + 1) % 100;
+    seen[j + 1] = 1;
+    S->angle_order.push_back((uint8_t)j);
+  }
+  for (int j = 101; j <= 200; j++) S->angle_order.push_back((uint8_t)(j - 1));
+  S->angle_hist.assign(200, 0);
+  if (S->order.ensure(200) || S->rej.ensure(200 * sizeof(unsigned))) return WZFH_ENOMEM;
+  if (S->outer2.ensure(sizeof kOuter.t2) || S->outer3.ensure(sizeof kOuter.t3) || S->posidx.ensure(16)) return WZFH_ENOMEM;
+  WZ_HIP(hipMemcpy(S->outer2.p, kOuter.t2, sizeof kOuter.t2, hipMemcpyHostToDevice));
+  WZ_HIP(hipMemcpy(S->outer3.p, kOuter.t3, sizeof kOuter.t3, hipMemcpyHostToDevice));
+  WZ_HIP(hipMemcpy(S->posidx.p, kOuter.pos_index, 16, hipMemcpyHostToDevice));
+  if (!P.rows.empty()) {
+    if (S->rows.ensure(P.rows.size() * sizeof(AbRow))) return WZFH_ENOMEM;
+    WZ_HIP(hipMemcpy(S->rows.p, P.rows.data(), P.rows.size() * sizeof(AbRow), hipMemcpyHostToDevice));
+  }
+  S->kab_host = build_kab(P.L1);
+  if (S->kab.ensure(S->kab_host.size())) return WZFH_ENOMEM;
+  WZ_HIP(hipMemcpy(S->kab.p, S->kab_host.data(), S->kab_host.size(), hipMemcpyHostToDevice));
+  build_stream_tables(P, S->base);
+  return WZFH_OK;
+}
+
+void device_free(wzfh_problem &P) {
+  DeviceState *S = P.dev;
+  if (!S) return;
+  for (DBuf *b : {&S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->posidx, &S->rows, &S->kab,
+                  &S->cell_res, &S->cell_rootok, &S->cut, &S->item_nodes, &S->item_leaves, &S->item_cands,
+                  &S->cands, &S->ctr, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
+                  &S->c_B, &S->c_scratch})
+    b->release();
+  for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+  delete S;
+  P.dev = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Stream: launch + exact post-processing.
+struct StreamOut {
+  std::vector<CandRec> cands;                 // sorted by (cell slot, path, ell)
+  std::vector<long long> cell_begin;          // per slot, into cands (ncells+1)
+};
+
+static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncells) {
+  const int half = P.n / 2;
+  int d0 = o.split_depth;
+  if (d0 <= 0) {
+    const bool capped = o.cand_cap > 0 || o.node_budget > 0;
+    if (capped) d0 = 3;
+    else d0 = ncells >= 4096 ? 2 : (ncells >= 256 ? 3 : 4);
+  }
+  return std::max(1, std::min(d0, std::max(1, half)));
+}
+
+static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o,
+                      wzfh_cell_stats *stats, StreamOut &out) {
+  DeviceState *S = P.dev;
+  StreamArgs a = S->base;
+  const int d0 = auto_split(P, o, ncells);
+  const int npaths = a.nch0 * (1 << (3 * (d0 - 1)));
+  int window = o.window > 0 ? o.window : ((o.cand_cap > 0 || o.node_budget > 0) ? std::min(npaths, 8) : npaths);
+  window = std::max(1, std::min(window, npaths));
+  a.d0 = d0; a.npaths = npaths; a.window = window; a.ncells = ncells;
+  a.node_budget = o.node_budget; a.cand_cap = o.cand_cap;
+  // per-cell residual words and root check
+  std::vector<uint64_t> res(2 * (size_t)ncells);
+  std::vector<int> rootok(ncells);
+  const int L = P.n, m = P.m;
+  for (int i = 0; i < ncells; i++) {
+    const HostCell &c = P.cells[cells[i]];
+    uint64_t rx = 0, ry = 0;
+    bool ok = true;
+    for (int q = 0; q < m; q++) {
+      const int tot = class_count(L, q, m);
+      rx |= (uint64_t)(c.px[q] + 16) << (5 * q);
+      ry |= (uint64_t)(c.py[q] + 16) << (5 * q);
+      if (std::abs(c.px[q]) > tot || std::abs(c.py[q]) > tot || ((c.px[q] + tot) & 1) || ((c.py[q] + tot) & 1)) ok = false;
+    }
+    res[2 * i] = rx; res[2 * i + 1] = ry; rootok[i] = ok;
+  }
+  const size_t nitems = (size_t)ncells * npaths;
+  if (S->cell_res.ensure(res.size() * 8) || S->cell_rootok.ensure(ncells * 4) || S->cut.ensure(ncells * 4) ||
+      S->item_nodes.ensure(nitems * 8) || S->item_leaves.ensure(nitems * 4) || S->item_cands.ensure(nitems * 4) ||
+      S->ctr.ensure(64))
+    return WZFH_ENOMEM;
+  unsigned long long cap = o.cand_cap > 0 ? (unsigned long long)ncells * (unsigned long long)(o.cand_cap + 512)
+                                          : (unsigned long long)std::max<size_t>(1 << 20, S->cands.bytes / sizeof(CandRec));
+  for (int attempt = 0; attempt < 8; attempt++) {
+    if (S->cands.ensure(cap * sizeof(CandRec))) return WZFH_ENOMEM;
+    WZ_HIP(hipMemcpyAsync(S->cell_res.p, res.data(), res.size() * 8, hipMemcpyHostToDevice, S->stream));
+    WZ_HIP(hipMemcpyAsync(S->cell_rootok.p, rootok.data(), ncells * 4, hipMemcpyHostToDevice, S->stream));
+    WZ_HIP(hipMemcpyAsync(S->order.p, S->angle_order.data(), 200, hipMemcpyHostToDevice, S->stream));
+    WZ_HIP(hipMemsetAsync(S->cut.p, 0x7F, ncells * 4, S->stream));
+    WZ_HIP(hipMemsetAsync(S->item_nodes.p, 0, nitems * 8, S->stream));
+    WZ_HIP(hipMemsetAsync(S->item_leaves.p, 0, nitems * 4, S->stream));
+    WZ_HIP(hipMemsetAsync(S->item_cands.p, 0, nitems * 4, S->stream));
+    WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
+    WZ_HIP(hipMemsetAsync(S->rej.p, 0, 200 * sizeof(unsigned), S->stream));
+    a.cell_res = S->cell_res.as<uint64_t>();
+    a.cell_rootok = S->cell_rootok.as<int>();
+    a.hcos = S->hcos.as<double>(); a.hsin = S->hsin.as<double>();
+    a.angle_order = S->order.as<uint8_t>();
+    a.work_ctr = S->ctr.as<unsigned long long>();
+    a.ncands = S->ctr.as<unsigned long long>() + 1;
+    a.cut = S->cut.as<int>();
+    a.item_nodes = S->item_nodes.as<uint64_t>();
+    a.item_leaves = S->item_leaves.as<uint32_t>();
+    a.item_cands = S->item_cands.as<uint32_t>();
+    a.cands = S->cands.as<CandRec>();
+    a.cand_capacity = cap;
+    a.angle_rej = S->rej.as<unsigned>();
+    const unsigned long long work_items = (unsigned long long)((npaths + window - 1) / window) * ncells * window;
+    int blocks = (int)std::min<unsigned long long>((unsigned long long)S->num_cu * 8, (work_items + 255) / 256 + 1);
+    blocks = std::max(blocks, 1);
+    WZ_HIP(hipEventRecord(S->ev[0], S->stream));
+    WZ_HIP(launch_stream(a, blocks, S->stream));
+    WZ_HIP(hipEventRecord(S->ev[1], S->stream));
+    unsigned long long ctr[2];
+    WZ_HIP(hipMemcpyAsync(ctr, S->ctr.p, 16, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipStreamSynchronize(S->stream));
+    float ms = 0;
+    WZ_HIP(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
+    S->last_ms[0] = ms;
+    if (ctr[1] > cap) { cap = ctr[1] + (ctr[1] >> 2) + 1024; continue; }
+    // ---- copy back and order
+    std::vector<uint64_t> inodes(nitems);
+    std::vector<uint32_t> ileaves(nitems), icands(nitems);
+    std::vector<CandRec> cr(ctr[1]);
+    std::vector<unsigned> rej(200);
+    WZ_HIP(hipMemcpy(inodes.data(), S->item_nodes.p, nitems * 8, hipMemcpyDeviceToHost));
+    WZ_HIP(hipMemcpy(ileaves.data(), S->item_leaves.p, nitems * 4, hipMemcpyDeviceToHost));
+    WZ_HIP(hipMemcpy(icands.data(), S->item_cands.p, nitems * 4, hipMemcpyDeviceToHost));
+    if (ctr[1]) WZ_HIP(hipMemcpy(cr.data(), S->cands.p, ctr[1] * sizeof(CandRec), hipMemcpyDeviceToHost));
+    WZ_HIP(hipMemcpy(rej.data(), S->rej.p, 200 * sizeof(unsigned), hipMemcpyDeviceToHost));
+    // refine the angle order for the next launch (decision-invariant)
+    {
+      std::vector<unsigned long long> cnt(200, 0);
+      for (int t = 0; t < 200; t++) S->angle_hist[S->angle_order[t]] += rej[t];
+      std::vector<uint8_t> ord(S->angle_order);
+      std::stable_sort(ord.begin(), ord.end(), [&](uint8_t x, uint8_t y) { return S->angle_hist[x] > S->angle_hist[y]; });
+      S->angle_order = ord;
+    }
+    std::sort(cr.begin(), cr.end(), [](const CandRec &x, const CandRec &y) {
+      if (x.item != y.item) return x.item < y.item;
+      return x.ell < y.ell;
+    });
+    out.cands.clear();
+    out.cell_begin.assign(ncells + 1, 0);
+    long long units = 0;
+    size_t pos = 0;
+    for (int cs = 0; cs < ncells; cs++) {
+      out.cell_begin[cs] = (long long)out.cands.size();
+      // item offsets in DFS order
+      std::vector<uint64_t> noff(npaths + 1, 0);
+      std::vector<uint64_t> loff(npaths + 1, 0);
+      for (int p = 0; p < npaths; p++) {
+        noff[p + 1] = noff[p] + inodes[(size_t)cs * npaths + p];
+        loff[p + 1] = loff[p] + ileaves[(size_t)cs * npaths + p];
+      }
+      const uint64_t total_nodes = noff[npaths];
+      const uint32_t lo = (uint32_t)cs * npaths, hi = lo + npaths;
+      while (pos < cr.size() && cr[pos].item < lo) pos++;
+      long long taken = 0;
+      uint64_t last_g = 0;
+      uint32_t last_item = 0, last_leaf = 0;
+      bool cut_c = false, cut_n = false;
+      size_t q = pos;
+      for (; q < cr.size() && cr[q].item < hi; q++) {
+        const int p = (int)(cr[q].item - lo);
+        const uint64_t g = noff[p] + cr[q].ell;
+        if (o.node_budget > 0 && g >= (uint64_t)o.node_budget) { cut_n = true; break; }
+        if (o.cand_cap > 0 && taken >= o.cand_cap) { cut_c = true; break; }
+        out.cands.push_back(cr[q]);
+        taken++;
+        last_g = g; last_item = cr[q].item; last_leaf = cr[q].leaf;
+      }
+      while (q < cr.size() && cr[q].item < hi) q++;
+      pos = q;
+      wzfh_cell_stats &stt = stats[cs];
+      stt.cand = taken;
+      stt.hall_ok = taken;
+      stt.reserved = 0;
+      if (o.cand_cap > 0 && taken >= o.cand_cap) {
+        // the reference stops right after the cap-th candidate's leaf (rec() entry check)
+        stt.capped = 1;
+        stt.dfs_nodes = (long long)last_g + 1;
+        stt.leaves = (long long)(loff[last_item - lo] + last_leaf + 1);
+      } else if (o.node_budget > 0 && (cut_n || total_nodes > (uint64_t)o.node_budget)) {
+        stt.capped = 1;
+        stt.dfs_nodes = o.node_budget;
+        stt.leaves = -1;
+      } else {
+        stt.capped = 0;
+        stt.dfs_nodes = (long long)total_nodes;
+        stt.leaves = (long long)loff[npaths];
+      }
+      (void)cut_c;
+      units += stt.dfs_nodes;
+    }
+    out.cell_begin[ncells] = (long long)out.cands.size();
+    S->last_units[0] = units;
+    return WZFH_OK;
+  }
+  set_error("candidate buffer kept overflowing");
+  return WZFH_ECAPACITY;
+}
+
+// ---------------------------------------------------------------------------
+static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                        wzfh_completion *out) {
+  DeviceState *S = P.dev;
+  if (ncand <= 0) return WZFH_OK;
+  std::vector<int> roff(ncand, -1), rlen(ncand, 0);
+  int maxrows = 0;
+  for (int i = 0; i < ncand; i++) {
+    const int c = cell_of ? cell_of[i] : -1;
+    if (c < 0 || !P.ab_prof) continue;
+    if (c >= (int)P.cells.size()) { set_error("cell index out of range"); return WZFH_EINVAL; }
+    const int li = P.cell_rows[c];
+    if (li < 0) { roff[i] = 0; rlen[i] = 0; continue; }   // proven-dead cell: no row can complete
+    if ((long long)P.row_len[li] > P.cfg.abp_cap) continue;   // oversized list: unconstrained
+    roff[i] = P.row_off[li];
+    rlen[i] = P.row_len[li];
+    maxrows = std::max(maxrows, rlen[i]);
+  }
+  const int words = std::max(1, (maxrows + 63) / 64);
+  const int blocks = std::max(1, std::min(S->num_cu * 8, (ncand + 3) / 4));
+  const size_t scratch = (size_t)blocks * 4 * 32 * words * 2 * sizeof(uint64_t);
+  if (S->c_cd.ensure((size_t)ncand * 16) || S->c_roff.ensure(ncand * 4) || S->c_rlen.ensure(ncand * 4) ||
+      S->c_verdict.ensure(ncand * 4) || S->c_nodes.ensure(ncand * 8) || S->c_A.ensure(ncand * 8) ||
+      S->c_B.ensure(ncand * 8) || S->c_scratch.ensure(scratch) || S->ctr.ensure(64))
+    return WZFH_ENOMEM;
+  WZ_HIP(hipMemcpyAsync(S->c_cd.p, cd, (size_t)ncand * 16, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_roff.p, roff.data(), ncand * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_rlen.p, rlen.data(), ncand * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
+  CompleteArgs a{};
+  a.n = P.n; a.L = P.L1; a.m = P.m;
+  a.absA = std::abs(P.cfg.a); a.absB = std::abs(P.cfg.b);
+  a.outer_k = P.cfg.outer_k; a.canon = !P.cfg.no_canon_ab;
+  a.ncand = ncand; a.budget = budget;
+  a.cd = S->c_cd.as<uint64_t>();
+  a.row_off = S->c_roff.as<int>(); a.row_len = S->c_rlen.as<int>();
+  a.rows = S->rows.p ? S->rows.as<uint64_t>() : S->c_cd.as<uint64_t>();
+  for (int c = 0; c < 8; c++) a.tic[c] = c < P.m ? class_count(P.L1, c, P.m) : 0;
+  a.outer2 = S->outer2.as<uint8_t>(); a.outer3 = S->outer3.as<uint8_t>();
+  a.pos_index = S->posidx.as<int8_t>();
+  a.scratch = S->c_scratch.as<uint64_t>(); a.scratch_words = words;
+  a.work_ctr = S->ctr.as<unsigned long long>();
+  a.verdict = S->c_verdict.as<int>(); a.nodes = S->c_nodes.as<long long>();
+  a.outA = S->c_A.as<uint64_t>(); a.outB = S->c_B.as<uint64_t>();
+  WZ_HIP(hipEventRecord(S->ev[2], S->stream));
+  WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, S->stream));
+  WZ_HIP(hipEventRecord(S->ev[3], S->stream));
+  std::vector<int> v(ncand);
+  std::vector<long long> nd(ncand);
+  std::vector<uint64_t> A(ncand), B(ncand);
+  WZ_HIP(hipMemcpyAsync(v.data(), S->c_verdict.p, ncand * 4, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(nd.data(), S->c_nodes.p, ncand * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(A.data(), S->c_A.p, ncand * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(B.data(), S->c_B.p, ncand * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipStreamSynchronize(S->stream));
+  float ms = 0;
+  WZ_HIP(hipEventElapsedTime(&ms, S->ev[2], S->ev[3]));
+  S->last_ms[1] = ms;
+  long long units = 0;
+  bool internal = false;
+  for (int i = 0; i < ncand; i++) {
+    out[i].verdict = v[i];
+    out[i].reserved = 0;
+    out[i].nodes = nd[i];
+    out[i].A = A[i];
+    out[i].B = B[i];
+    units += nd[i];
+    if (v[i] == 4) internal = true;
+  }
+  S->last_units[1] = units;
+  if (internal) { set_error("completer FOUND failed the exact NPAF recheck"); return WZFH_EINTERNAL; }
+  return WZFH_OK;
+}
+
+}  // namespace wzfh
+
+using namespace wzfh;
+
+extern "C" {
+
+const char *wzfh_last_error(void) { return g_err.c_str(); }
+
+int wzfh_config_default(wzfh_config *cfg, int n, int a, int b, int c, int d) {
+  if (!cfg) return WZFH_EINVAL;
+  std::memset(cfg, 0, sizeof *cfg);
+  cfg->n = n; cfg->a = a; cfg->b = b; cfg->c = c; cfg->d = d;
+  cfg->m6 = n >= 36;
+  cfg->thm211b = n >= 36;
+  cfg->thm212 = n >= 36;
+  cfg->ab_prof = -1;
+  cfg->outer_k = 3;
+  cfg->abp_cap = 20000;
+  return WZFH_OK;
+}
+
+static int init_impl(const wzfh_config *cfg, wzfh_problem **out, bool with_device) {
+  if (!cfg || !out) return WZFH_EINVAL;
+  *out = nullptr;
+  const int n = cfg->n;
+  if (n < 4) { set_error("n must be >= 4"); return WZFH_EINVAL; }
+  if (n + 1 > kMaxL - 1) { set_error("n too large for one 64-bit word per sequence"); return WZFH_EINVAL; }
+  if (cfg->a * cfg->a + cfg->b * cfg->b + cfg->c * cfg->c + cfg->d * cfg->d != 4 * n + 2) {
+    set_error("sig norm != 4n+2");
+    return WZFH_EINVAL;
+  }
+  if (cfg->outer_k != 0 && cfg->outer_k != 2 && cfg->outer_k != 3) { set_error("outer_k must be 0, 2 or 3"); return WZFH_EINVAL; }
+  wzfh_problem *P = new wzfh_problem();
+  P->cfg = *cfg;
+  if (P->cfg.abp_cap <= 0) P->cfg.abp_cap = 20000;
+  P->n = n; P->L1 = n + 1; P->m = cfg->m6 ? 6 : 3;
+  if (n >= 36) { P->cfg.thm211b = 1; P->cfg.thm212 = 1; }   // wz_match.cpp:1388
+  P->pinC = cfg->c == 0; P->pinD = cfg->d == 0;
+  if (cfg->canon && (P->pinC || P->pinD)) { P->pinC = P->pinD = false; P->pins_dropped = true; }   // :1661-1662
+  P->ab_prof = cfg->ab_prof < 0 ? (cfg->m6 != 0) : (cfg->ab_prof != 0);
+  int rc = build_problem(*P);
+  if (rc == WZFH_OK && with_device) rc = device_init(*P);
+  if (rc != WZFH_OK) { device_free(*P); delete P; return rc; }
+  *out = P;
+  return WZFH_OK;
+}
+
+int wzfh_init(const wzfh_config *cfg, wzfh_problem **out) { return init_impl(cfg, out, true); }
+
+int wzfh_init_host(const wzfh_config *cfg, wzfh_problem **out) { return init_impl(cfg, out, false); }
+
+void wzfh_destroy(wzfh_problem *p) {
+  if (!p) return;
+  device_free(*p);
+  delete p;
+}
+
+int wzfh_num_cells(const wzfh_problem *p) { return p ? (int)p->cells.size() : 0; }
+
+int wzfh_cell_info(const wzfh_problem *p, int pi, int *px, int *py, int *kept, int *rows) {
+  if (!p || pi < 0 || pi >= (int)p->cells.size()) return WZFH_EINVAL;
+  for (int c = 0; c < p->m; c++) { if (px) px[c] = p->cells[pi].px[c]; if (py) py[c] = p->cells[pi].py[c]; }
+  if (kept) *kept = p->kept[pi];
+  if (rows) *rows = !p->ab_prof ? -2 : (p->cell_rows[pi] < 0 ? -1 : p->row_len[p->cell_rows[pi]]);
+  return WZFH_OK;
+}
+
+int wzfh_order_digest(const wzfh_problem *p, char *buf, int buflen) {
+  if (!p || !buf || buflen < 34) return WZFH_EINVAL;
+  std::snprintf(buf, buflen, "%s", p->digest.c_str());
+  return WZFH_OK;
+}
+
+int wzfh_stream_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_stream_opts *opts,
+                      wzfh_cell_stats *stats, uint64_t *cd, long long cd_capacity, long long *offsets) {
+  if (!p || !cells || ncells <= 0 || !stats || !offsets) return WZFH_EINVAL;
+  for (int i = 0; i < ncells; i++)
+    if (cells[i] < 0 || cells[i] >= (int)p->cells.size()) { set_error("cell index out of range"); return WZFH_EINVAL; }
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  wzfh_stream_opts o{};
+  if (opts) o = *opts;
+  StreamOut so;
+  int rc = run_stream(*p, cells, ncells, o, stats, so);
+  if (rc != WZFH_OK) return rc;
+  for (int i = 0; i <= ncells; i++) offsets[i] = so.cell_begin[i];
+  const long long total = so.cell_begin[ncells];
+  if (cd) {
+    if (total > cd_capacity) return WZFH_ECAPACITY;
+    for (long long j = 0; j < total; j++) { cd[2 * j] = so.cands[j].X; cd[2 * j + 1] = so.cands[j].Y; }
+  }
+  return WZFH_OK;
+}
+
+int wzfh_complete(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                  wzfh_completion *out) {
+  if (!p || (ncand > 0 && (!cd || !out)) || ncand < 0 || budget < 0) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  return run_complete(*p, cd, cell_of, ncand, budget, out);
+}
+
+long long wzfh_flat_score(int n, uint64_t C, uint64_t D) {
+  long long sc = 0;
+  for (int s = 1; s < n; s++) {
+    const uint64_t mk = (1ull << (n - s)) - 1;
+    const int dis = __builtin_popcountll((C ^ (C >> s)) & mk) + __builtin_popcountll((D ^ (D >> s)) & mk);
+    sc += std::abs(2 * (n - s) - 2 * dis);
+  }
+  return sc;
+}
+
+int wzfh_npaf_check(int n, const int8_t *A, const int8_t *B, const int8_t *C, const int8_t *D, int *max_abs) {
+  if (!A || !B || !C || !D || n < 1) return WZFH_EINVAL;
+  int mx = 0;
+  const int n1 = n + 1;
+  for (int s = 1; s <= n; s++) {
+    int v = 0;
+    for (int i = 0; i + s < n1; i++) v += A[i] * A[i + s] + B[i] * B[i + s];
+    for (int i = 0; i + s < n; i++) v += C[i] * C[i + s] + D[i] * D[i + s];
+    mx = std::max(mx, std::abs(v));
+  }
+  if (max_abs) *max_abs = mx;
+  return WZFH_OK;
+}
+
+int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units) {
+  if (!p || !p->dev || kind < 0 || kind > 1) return WZFH_EINVAL;
+  if (ms) *ms = p->dev->last_ms[kind];
+  if (units) *units = p->dev->last_units[kind];
+  return WZFH_OK;
+}
+
+int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts,
+                      wzfh_search_result *out) {
+  if (!p || !cells || ncells <= 0 || !opts || !out) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  for (int i = 0; i < ncells; i++)
+    if (cells[i] < 0 || cells[i] >= (int)p->cells.size()) { set_error("cell index out of range"); return WZFH_EINVAL; }
+  const int n = p->n;
+  const int K = opts->top_k;
+  auto one_batch = [&](const int *cl, int nc, wzfh_search_result *res) -> int {
+    std::vector<wzfh_cell_stats> st(nc);
+    StreamOut so;
+    int rc = run_stream(*p, cl, nc, opts->stream, st.data(), so);
+    if (rc != WZFH_OK) return rc;
+    // per cell: stable sort the streamed candidates by flat score, take the top K
+    std::vector<uint64_t> cd;
+    std::vector<int> cellof, owner;
+    std::vector<long long> rank, sidx;
+    for (int i = 0; i < nc; i++) {
+      const long long b = so.cell_begin[i], e = so.cell_begin[i + 1];
+      std::vector<std::pair<long long, long long>> key;
+      key.reserve(e - b);
+      for (long long j = b; j < e; j++) key.push_back({wzfh_flat_score(n, so.cands[j].X, so.cands[j].Y), j - b});
+      std::stable_sort(key.begin(), key.end(), [](auto &x, auto &y) { return x.first < y.first; });
+      const long long take = std::min<long long>(K, (long long)key.size());
+      for (long long r = 0; r < take; r++) {
+        const CandRec &c = so.cands[b + key[r].second];
+        cd.push_back(c.X); cd.push_back(c.Y);
+        cellof.push_back(cl[i]); owner.push_back(i); rank.push_back(r); sidx.push_back(key[r].second);
+      }
+      res[i] = wzfh_search_result{};
+      res[i].stream = st[i];
+      res[i].hit_rank = -1; res[i].hit_idx = -1;
+    }
+    const int nk = (int)owner.size();
+    std::vector<wzfh_completion> comp(nk);
+    const double stream_ms = p->dev->last_ms[0];
+    rc = run_complete(*p, cd.data(), cellof.data(), nk, opts->budget, comp.data());
+    p->dev->last_ms[0] = stream_ms;
+    if (rc != WZFH_OK && rc != WZFH_EINTERNAL) return rc;
+    for (int t = 0; t < nk; t++) {
+      wzfh_search_result &r = res[owner[t]];
+      r.completed++;
+      r.ab_nodes += comp[t].nodes;
+      if (comp[t].verdict == 2) r.clean_no++;
+      if (comp[t].verdict == 3) r.aborted++;
+      if (comp[t].verdict == 0 && !r.hit) {
+        r.hit = 1; r.hit_rank = rank[t]; r.hit_idx = sidx[t];
+        r.A = comp[t].A; r.B = comp[t].B; r.C = cd[2 * t]; r.D = cd[2 * t + 1];
+      }
+    }
+    return rc;
+  };
+  if (opts->batched) return one_batch(cells, ncells, out);
+  double sms = 0, cms = 0;
+  long long su = 0, cu = 0;
+  for (int i = 0; i < ncells; i++) {
+    int rc = one_batch(cells + i, 1, out + i);
+    if (rc != WZFH_OK) return rc;
+    sms += p->dev->last_ms[0]; cms += p->dev->last_ms[1];
+    su += p->dev->last_units[0]; cu += p->dev->last_units[1];
+  }
+  p->dev->last_ms[0] = sms; p->dev->last_ms[1] = cms;
+  p->dev->last_units[0] = su; p->dev->last_units[1] = cu;
+  return WZFH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// wzfh_internal.h -- shared host-side structures of libwzfh.so.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wzfh.h"
+
+namespace wzfh {
+
+constexpr int kMaxL = 64;          // one 64-bit word per sequence
+constexpr int kMaxDepth = 32;
+
+struct HostCell {
+  int8_t px[8];
+  int8_t py[8];
+};
+
+// One allowed A,B mod-m class-sum row (AbpRow, wz_match.cpp:970).
+struct AbRow {
+  int8_t k[8];
+  int8_t r[8];
+};
+
+struct DeviceState;                // wzfh_device.hip
+
+// Thm 2.2 quad tables (wz_match.cpp:531-545): index i is the sign pattern
+// (i&8, i&4, i&2, i&1) of (X[d], Y[d], X[L-1-d], Y[L-1-d]); POS/NEG keep ascending i.
+struct Quads {
+  int all[16][4], pos[8][4], neg[8][4];
+  Quads();
+};
+extern const Quads kQuads;
+
+// A1 outer-lag tables (wz_match.cpp:547-598), built from their definition.
+struct OuterTables {
+  uint8_t t2[16][5][9];
+  uint8_t t3[128][5][9][13];
+  int8_t pos_index[16];
+  OuterTables();
+};
+extern const OuterTables kOuter;
+
+std::string set_error(const std::string &msg);   // records the thread's last error
+
+}  // namespace wzfh
+
+struct wzfh_problem {
+  wzfh_config cfg;
+  int n = 0, L1 = 0, m = 6;
+  bool pinC = false, pinD = false, pins_dropped = false, ab_prof = false;
+  std::vector<wzfh::HostCell> cells;
+  std::vector<uint8_t> kept;
+  std::vector<int> cell_rows;          // per cell: index into row_off/row_len, -1 = dead
+  std::vector<int> row_off, row_len;   // per row list
+  std::vector<wzfh::AbRow> rows;
+  std::string digest;                  // order-3 "<list>:<kept>"
+  double hcos[201][wzfh::kMaxL];
+  double hsin[201][wzfh::kMaxL];
+  wzfh::DeviceState *dev = nullptr;
+};
+
+namespace wzfh {
+// Host setup (wzfh_setup.cpp).
+int build_problem(wzfh_problem &P);
+// Device side (wzfh_device.hip).
+int device_init(wzfh_problem &P);
+void device_free(wzfh_problem &P);
+}  // namespace wzfh

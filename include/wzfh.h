@@ -1,0 +1,182 @@
+/*
+ * wzfh.h -- C ABI of the MI355X BS(n+1,n) first-hit engine (libwzfh.so).
+ *
+ * Drop-in boundary for the reference solver's WZ_FIRSTHIT path
+ * (src/solver/wz_match.cpp:1529-2892). The reference is a process CLI; its
+ * in-process seams are count_pairs22 (:678-784), fh_complete_ab (:1298-1341),
+ * hall_ok* (:116-181), npaf_at (:184-192) and autocorr_key (:855-871). Each entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Plain C types only. Sequences cross the boundary either as int8 arrays of
+ *     +1/-1 (the reference's int arrays) or as packed 64-bit words with bit i set
+ *     when element i is -1 (n <= 63 on this path).
+ *   - Every call returns an int status (WZFH_OK or a negative WZFH_E* code); no
+ *     call exits the process. wzfh_last_error() gives the text of the last error
+ *     on the calling thread.
+ *   - Caller-owned buffers are borrowed for the duration of the call only.
+ *     Device memory is owned by the problem handle.
+ *   - A problem handle is not reentrant: calls on one handle must not overlap
+ *     (the reference's completer and stream are non-reentrant too, :951-975).
+ *   - The library needs a HIP device; without one wzfh_init fails with
+ *     WZFH_ENODEV. There is no CPU fallback.
+ */
+#ifndef WZFH_H
+#define WZFH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WZFH_OK          0
+#define WZFH_EINVAL     -1   /* bad argument (reference exit 1/2 cases)            */
+#define WZFH_ENOMEM     -2   /* host or device allocation failed                   */
+#define WZFH_EDEVICE    -3   /* a HIP call failed                                  */
+#define WZFH_ENODEV     -4   /* no HIP device                                      */
+#define WZFH_ECAPACITY  -5   /* caller buffer too small (required size reported)   */
+#define WZFH_EINTERNAL  -6   /* completer FOUND failed the NPAF recheck (exit 5)   */
+#define WZFH_ETRUNC     -7   /* cell list hit the 20M cap (reference exit 2)       */
+
+/* Verdicts of one A,B completion (wz_match.cpp:1295-1297, 1330-1335). */
+#define WZFH_HIT        0
+#define WZFH_PREREJECT  1
+#define WZFH_CLEAN_NO   2
+#define WZFH_ABORTED    3
+#define WZFH_INTERNAL   4
+
+/* Problem configuration: argv <n> <a> <b> <c> <d> plus the stream-shaping WZ_* flags
+ * of wz_match.cpp:1363-1388, 1554-1891, 2119-2169. */
+typedef struct {
+  int n, a, b, c, d;
+  int m6;            /* WZ_FH_M6: 1 = mod-6 cells (default n>=36), 0 = mod-3       */
+  int thm211b;       /* WZ_THM211B (forced on for n >= 36)                          */
+  int thm212;        /* WZ_THM212  (forced on for n >= 36)                          */
+  int canon;         /* WZ_FH_ORBIT_CANON                                           */
+  int orbit_q;       /* WZ_FH_ORBIT_Q (1 = where proven safe, 2 = forced)          */
+  int endpos;        /* WZ_FH_ENDPOS                                                */
+  int order;         /* WZ_FH_PROF_ORDER 0..3                                       */
+  int rev;           /* WZ_FH_STREAM_REV                                            */
+  int ab_prof;       /* WZ_FH_AB_PROF (-1 = default: on for mod-6 cells)           */
+  int outer_k;       /* WZ_FH_OUTER_K (0, 2, 3; default 3)                          */
+  int no_canon_ab;   /* WZ_FH_NO_CANON                                              */
+  long long abp_cap; /* WZ_FH_AB_PROF_CAP (default 20000)                           */
+} wzfh_config;
+
+typedef struct wzfh_problem wzfh_problem;
+
+/* Fills the reference defaults for a signature (wz_match.cpp:1355-1388, 1561). */
+int wzfh_config_default(wzfh_config *cfg, int n, int a, int b, int c, int d);
+
+/* Per-(n, class) setup: norm/autocorr sets, the profile-cell list with ENDPOS,
+ * ordering and orbit canonicalization, the A,B profile rows, the hall tables, and
+ * their device copies. Replaces wz_match.cpp:1391-1410, 1514-1522, 1588-1891,
+ * 2318-2381. Returns WZFH_EINVAL for a bad signature norm (reference exit 1). */
+int wzfh_init(const wzfh_config *cfg, wzfh_problem **out);
+/* Host-only setup: the same per-(n, class) state without touching a device, for
+ * inspecting the cell list (the reference's WZ_FH_LIST_ONLY, wz_match.cpp:1873).
+ * Stream / complete / search calls on such a handle return WZFH_ENODEV. */
+int wzfh_init_host(const wzfh_config *cfg, wzfh_problem **out);
+void wzfh_destroy(wzfh_problem *p);
+const char *wzfh_last_error(void);
+
+int wzfh_num_cells(const wzfh_problem *p);
+/* Cell pi of the ordered list: px, py (m ints each), kept (orbit-canon keep flag),
+ * rows (A,B profile rows of the cell; -1 = proven dead, -2 = rows off). */
+int wzfh_cell_info(const wzfh_problem *p, int pi, int *px, int *py, int *kept, int *rows);
+/* 64-bit FNV digests "<list>:<kept>" of the ordered cell list under order 3
+ * (wz_match.cpp:1834-1856); empty string otherwise. buf needs 34 bytes. */
+int wzfh_order_digest(const wzfh_problem *p, char *buf, int buflen);
+
+/* ---- C,D cell stream (count_pairs22 + hall_ok_pair_fast, wz_match.cpp:678-784,
+ * 143-168, on the C,D side of a mod-m cell) ------------------------------------ */
+typedef struct {
+  long long cand_cap;     /* stop a cell after this many candidates (WZ_FH_CELLSIZE /
+                             BUF_CAP front); 0 = whole cell                          */
+  long long node_budget;  /* stop a cell after this many DFS nodes (a build-side
+                             serving policy, no reference counterpart); 0 = none     */
+  int split_depth;        /* frontier depth of the subtree split; 0 = autotuned      */
+  int window;             /* subtrees of one cell in flight; 0 = autotuned           */
+} wzfh_stream_opts;
+
+typedef struct {
+  long long cand;         /* candidates streamed (CELLSIZE cand=)                    */
+  long long leaves;       /* exact-class-sum leaves (CELLSIZE leaves=); -1 when a
+                             node budget cut the cell                                */
+  long long hall_ok;      /* leaves passing hall_ok (CELLSIZE hall_ok=)             */
+  long long dfs_nodes;    /* DFS node entries (CDSTAT dfs_nodes=)                    */
+  int capped;             /* 1 when a cap or budget cut the cell                     */
+  int reserved;
+} wzfh_cell_stats;
+
+/* Streams cells[0..ncells) (indices into the ordered list) on the GPU. The candidates
+ * of cell i land in DFS order in cd[2*offsets[i] .. 2*offsets[i+1]) as pairs of
+ * packed words (C, D). offsets has ncells+1 entries. cd may be NULL to count only.
+ * Returns WZFH_ECAPACITY (and the needed count in offsets[ncells]) when
+ * cd_capacity (in candidates) is too small. */
+int wzfh_stream_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_stream_opts *opts,
+                      wzfh_cell_stats *stats, uint64_t *cd, long long cd_capacity,
+                      long long *offsets);
+
+/* ---- A,B completion (fh_complete_ab, wz_match.cpp:1298-1341) ----------------- */
+typedef struct {
+  int verdict;            /* WZFH_HIT .. WZFH_INTERNAL                               */
+  int reserved;
+  long long nodes;        /* fh_nodes_total delta (charged + odd-L middle nodes)     */
+  uint64_t A, B;          /* the first completion found (verdict 0)                  */
+} wzfh_completion;
+
+/* Completes ncand candidates (packed C,D word pairs) with a node budget each
+ * (0 = exact). cell_of[i] >= 0 applies that cell's A,B profile rows (the in-stream
+ * completer); -1 runs unconstrained (the WZ_FH_COMPLETE_C/_D mode). cell_of may be
+ * NULL (all unconstrained). Returns WZFH_EINTERNAL if any verdict is 4. */
+int wzfh_complete(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand,
+                  long long budget, wzfh_completion *out);
+
+/* ---- Front-only cell search: stream -> flatness top-K -> complete ------------
+ * The unit of the reference's front policy (WZ_FH_BUF_CAP=cand_cap,
+ * WZ_FH_DRAIN_TOP=top_k, WZ_FH_DRAIN_BATCHES=1; wz_match.cpp:2656-2714), batched over
+ * many cells in one persistent launch. Per cell: the first cand_cap candidates (and
+ * at most node_budget DFS nodes) are stable-sorted by flat_score, the first top_k
+ * are completed with `budget` nodes each under the cell's profile rows. */
+typedef struct {
+  wzfh_stream_opts stream;
+  int top_k;
+  int batched;            /* 1 = all cells in one persistent launch, 0 = one cell per
+                             launch (the plain path)                                 */
+  long long budget;       /* A,B node budget per candidate                           */
+} wzfh_search_opts;
+
+typedef struct {
+  wzfh_cell_stats stream;
+  int completed;          /* candidates completed (<= top_k)                         */
+  int hit;                /* 1 if a completion hit                                   */
+  long long hit_rank;     /* sorted rank of the first hit (-1 none)                  */
+  long long hit_idx;      /* in-cell stream index of the first hit (-1 none)         */
+  long long ab_nodes;     /* total completer nodes of the cell                       */
+  int clean_no, aborted;
+  uint64_t A, B, C, D;    /* the hit (if any)                                        */
+} wzfh_search_result;
+
+int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts,
+                      wzfh_search_result *out);
+
+/* ---- Exact checks ----------------------------------------------------------- */
+/* Summed NPAF over s = 1..n of int8 +1/-1 sequences (npaf_at, wz_match.cpp:184-192,
+ * and the VERIFY line :2640-2646). *max_abs gets max_s |NPAF[s]|. */
+int wzfh_npaf_check(int n, const int8_t *A, const int8_t *B, const int8_t *C, const int8_t *D,
+                    int *max_abs);
+/* Sum_s |CD[s]| of packed C,D (the flat_score lambda, wz_match.cpp:2569-2583). */
+long long wzfh_flat_score(int n, uint64_t C, uint64_t D);
+
+/* ---- Device timing (for measurement) ----------------------------------------- */
+/* Average device duration of the named kernel over the last wzfh_search_cells /
+ * wzfh_stream_cells / wzfh_complete call, measured with HIP events on the stream the
+ * kernel runs on. kind: 0 = stream kernel, 1 = completer kernel. */
+int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,79 @@
+/*
+ * wz_oracle.h -- CPU oracle for the BS(n+1,n) first-hit hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain, scalar C++ restatement of the
+ * reference solver's WZ_FIRSTHIT path (src/solver/wz_match.cpp), used as the
+ * checker for the HIP library and as the CPU baseline in bench.py. The shipped
+ * library never links, loads or calls it; only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg do.
+ *
+ * Every function cites the reference lines it restates. Pinning (how each part is
+ * checked against something other than itself):
+ *   - cell list, orbit canon, order      : golden per-cell fixtures from the
+ *                                          compiled reference (stream_cells.json)
+ *   - C,D stream, hall_ok, DFS node count: same fixtures (counts + stream SHA-1)
+ *   - completer (verdict, nodes, A,B)    : complete_only.json (six known-answer
+ *                                          controls, n=44 archive candidates,
+ *                                          budget aborts) + firsthit_runs.json
+ *   - npaf / flat score                  : closed form + 43 banked solutions
+ *   - outer (A1) tables                  : brute-force enumeration in tests
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int n, a, b, c, d;   /* signature as given on the reference command line  */
+  int m6;              /* 1: mod-6 cells (survive_profiles6), 0: mod-3 cells */
+  int thm211b, thm212; /* forced on at n >= 36 like wz_match.cpp:1388       */
+  int canon;           /* WZ_FH_ORBIT_CANON                                  */
+  int orbit_q;         /* WZ_FH_ORBIT_Q (1 = only where proven safe, 2 = force) */
+  int endpos;          /* WZ_FH_ENDPOS                                       */
+  int order;           /* WZ_FH_PROF_ORDER 0..3                              */
+  int rev;             /* WZ_FH_STREAM_REV                                   */
+  int ab_prof;         /* -1 = default (= m6)                                */
+  int outer_k;         /* WZ_FH_OUTER_K (default 3)                          */
+  int no_canon_ab;     /* WZ_FH_NO_CANON (completer root/reversal canon off) */
+  long long abp_cap;   /* WZ_FH_AB_PROF_CAP (default 20000)                  */
+  int no_cells;        /* 1: skip the cell list (completion-only use)        */
+} wzo_config;
+
+typedef struct {
+  long long leaves, hall_ok, dfs_nodes, cand;
+} wzo_stream_stats;
+
+void *wzo_create(const wzo_config *cfg);
+void wzo_destroy(void *h);
+int wzo_num_cells(void *h);
+/* px, py: m ints each. kept: orbit-canon keep flag (1 when canon is off).
+ * rows: number of A,B profile rows of the cell (-1 = proven dead, -2 = rows off). */
+int wzo_cell_info(void *h, int pi, int *px, int *py, int *kept, int *rows);
+/* Streams cell pi in DFS order; stops after `cap` candidates (cap <= 0: none).
+ * Writes up to out_cap candidates as int8 C[n] then D[n] per record.
+ * Returns the number of candidates streamed. */
+long long wzo_stream_cell(void *h, int pi, long long cap, int8_t *out, long long out_cap,
+                          wzo_stream_stats *st);
+/* One A,B completion. pi >= 0 applies cell pi's profile rows (when rows are on and the
+ * list is within the cap), pi < 0 runs unconstrained (WZ_FH_COMPLETE_C/_D mode).
+ * Returns the verdict 0..4; nodes = charged+middle nodes (fh_nodes_total delta). */
+int wzo_complete(void *h, int pi, const int8_t *C, const int8_t *D, long long budget,
+                 long long *nodes, int8_t *A, int8_t *B);
+long long wzo_flat_score(int n, const int8_t *C, const int8_t *D);
+/* Summed NPAF at shift s (wz_match.cpp:184-192). */
+int wzo_npaf_at(const int8_t *A, const int8_t *B, int n1, const int8_t *C, const int8_t *D,
+                int n2, int s);
+/* Front-only cell unit: stream the first `buf_cap` candidates of cell pi, stable-sort
+ * them by flat score, complete the first `top_k` with `budget`. verdicts/nodes get
+ * top_k entries (order = sorted rank); returns candidates streamed. */
+long long wzo_cell_front(void *h, int pi, long long buf_cap, int top_k, long long budget,
+                         int *verdicts, long long *nodes, long long *stream_idx,
+                         wzo_stream_stats *st);
+/* A1 outer tables (wz_match.cpp:560-598) for test cross-checks. */
+void wzo_outer_tables(uint8_t *outer2 /*16*5*9*/, uint8_t *outer3 /*128*5*9*13*/);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,278 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures in this folder by running the reference solver.
+
+The reference binary is oracle/_ref/wz_match, compiled by oracle/Makefile from the
+reference's own single source file (src/solver/wz_match.cpp) exactly as the
+reference's identity gates compile it (tools/test_cd_prune.py:41-45). It runs only
+in the container that holds the reference; the fixtures it writes are plain JSON
+(inputs and outputs), committed so the GPU host never needs the reference.
+
+Fixtures:
+  stream_cells.json   per-cell CELLSIZE/CDSTAT counts and a SHA-1 of each cell's
+                      WZ_FH_DUMP stream lines (n = 6..13 every class, canon and
+                      stream direction swept; n = 19 every class; ENDPOS cases)
+  complete_only.json  WZ_FH_COMPLETE_C/_D verdicts, node counts and first A,B for
+                      the six known-answer controls, sampled n = 44 candidates from
+                      results/archive/n44_cands_20k.txt and sampled n = 13/19/29
+                      stream candidates, including budget-abort cases
+  firsthit_runs.json  full WZ_FIRSTHIT runs (stream -> sort -> complete) at n = 6..13
+                      with WZ_FH_CAND_LOG: FIRSTHIT line, solution, summary counters
+                      and the per-completion (idx, ci, pi, r, nodes) sequence
+  firsthit_n29.json   the n = 29 known answer (WZ_FH_M6=1, default completer)
+
+Usage: python tests/golden/make_golden.py [--only NAME] [--jobs 8]
+"""
+import argparse
+import concurrent.futures as cf
+import hashlib
+import itertools
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parents[1]
+REF = REPO / 'oracle' / '_ref' / 'wz_match'
+REFTREE = Path('/root/reference')
+ENV0 = {k: v for k, v in os.environ.items() if not k.startswith(('WZ_', 'FH_'))}
+
+
+def classes(n):
+    """Signature classes a<=b, c<=d, nonnegative, parity-correct, norm 4n+2."""
+    out = []
+    for a, b, c, d in itertools.product(range(n + 2), range(n + 2), range(n + 1), range(n + 1)):
+        if a > b or c > d:
+            continue
+        if (a - (n + 1)) % 2 or (b - (n + 1)) % 2 or (c - n) % 2 or (d - n) % 2:
+            continue
+        if a * a + b * b + c * c + d * d == 4 * n + 2:
+            out.append((a, b, c, d))
+    return out
+
+
+def run(n, sig, settings, timeout=3600):
+    env = {**ENV0, **{k: str(v) for k, v in settings.items()}}
+    p = subprocess.run([str(REF), str(n), *map(str, sig)], env=env, text=True,
+                       capture_output=True, timeout=timeout)
+    return p.returncode, p.stdout
+
+
+CELL = re.compile(r'^CELLSIZE pi=(\d+) cand=(\d+) capped=([01]) partial=([01]) sec=\S+ '
+                  r'sec_at_buf=\S+ leaves=(\d+) hall_ok=(\d+)$', re.M)
+CDSTAT = re.compile(r'^CDSTAT pi=(\d+) dfs_nodes=(\d+)', re.M)
+SUMMARY_KEYS = ('candidates_streamed', 'pre_filter_rejected', 'backtracks_entered', 'clean_no_AB',
+                'budget_aborted', 'total_AB_nodes', 'cells_done', 'cells_prof_dead', 'cells_orbit_dup',
+                'cells_empty', 'cells_capped', 'pins_dropped', 'resume_pi', 'resume_batch', 'resume_k')
+
+
+def summary(out):
+    tail = out.split('=== FIRSTHIT SUMMARY')[-1]
+    d = {}
+    for k in SUMMARY_KEYS:
+        m = re.search(r'\b' + k + r'=(\d+)', tail)
+        if m:
+            d[k] = int(m[1])
+    m = re.search(r'^RESULT: (.*)$', tail, re.M)
+    d['result'] = m[1] if m else None
+    return d
+
+
+def stream_case(n, sig, canon, rev, endpos=0, order=1):
+    cfg = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_PROF_ORDER=order,
+               WZ_FH_ORBIT_CANON=canon, WZ_FH_STREAM_REV=rev, WZ_FH_PROG_SEC=99999,
+               WZ_FH_CELLSIZE=10 ** 12)
+    if endpos:
+        cfg['WZ_FH_ENDPOS'] = 1
+    with tempfile.TemporaryDirectory() as td:
+        dump = Path(td) / 'dump.txt'
+        rc, out = run(n, sig, {**cfg, 'WZ_FH_DUMP': dump})
+        lines = dump.read_text().splitlines(keepends=True)
+    assert rc in (0, 3), (n, sig, rc, out[-400:])
+    cells = [tuple(int(g) for g in m.groups()) for m in CELL.finditer(out)]
+    nodes = {int(m[1]): int(m[2]) for m in CDSTAT.finditer(out)}
+    rows, pos = [], 0
+    for pi, cand, capped, partial, leaves, hok in cells:
+        h = hashlib.sha1(''.join(lines[pos:pos + cand]).encode()).hexdigest()
+        pos += cand
+        rows.append([pi, cand, leaves, hok, nodes[pi], h])
+    assert pos == len(lines)
+    m = re.search(r'\[firsthit\] profiles=(\d+)', out)
+    kept = re.search(r'kept_orbits=(\d+)', out)
+    rec = dict(n=n, sig=list(sig), canon=canon, rev=rev, endpos=endpos, order=order,
+               profiles=int(m[1]), kept=int(kept[1]) if kept else None, cells=rows,
+               summary=summary(out))
+    if n <= 8:   # tiny: keep the literal stream for debugging
+        rec['dump'] = ''.join(lines)
+    return rec
+
+
+def gen_stream(jobs):
+    tasks = []
+    for n in range(6, 14):
+        for sig in classes(n):
+            for canon, rev in itertools.product((0, 1), (0, 1)):
+                tasks.append((n, sig, canon, rev, 0, 1))
+    for sig in classes(19):
+        tasks.append((19, sig, 1, 0, 0, 1))
+    tasks.append((19, classes(19)[0], 1, 1, 0, 1))
+    tasks.append((19, classes(19)[0], 0, 0, 0, 0))
+    for n in (12, 13):
+        for sig in classes(n)[:3]:
+            tasks.append((n, sig, 0, 0, 1, 1))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: stream_case(*t), tasks))
+    (HERE / 'stream_cells.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'stream_cells.json: {len(recs)} runs, {sum(len(r["cells"]) for r in recs)} cells, '
+          f'{sum(r["summary"]["candidates_streamed"] for r in recs)} candidates')
+
+
+def read_solution(path):
+    rows = [l.split() for l in path.read_text().splitlines() if l.strip() and not l.startswith('#')]
+    n = int(rows[0][0])
+    return n, [int(x) for x in rows[1][:4]], [[int(x) for x in r] for r in rows[2:6]]
+
+
+def complete_case(n, sig, C, D, budget, outer_k=3, m6=1):
+    cfg = dict(WZ_FIRSTHIT=1, WZ_FH_M6=m6, WZ_FH_AB_BUDGET=budget, WZ_FH_OUTER_K=outer_k,
+               WZ_FH_COMPLETE_C=','.join(map(str, C)), WZ_FH_COMPLETE_D=','.join(map(str, D)))
+    rc, out = run(n, sig, cfg)
+    m = re.search(r'^COMPLETE_ONLY r=(\d) nodes=(\d+)', out, re.M)
+    assert m, (n, sig, out[-300:])
+    rec = dict(n=n, sig=list(sig), C=C, D=D, budget=budget, outer_k=outer_k, r=int(m[1]), nodes=int(m[2]))
+    if rec['r'] == 0:
+        rec['A'] = [int(x) for x in re.search(r'^A = \{([^}]*)\}', out, re.M)[1].split(',')]
+        rec['B'] = [int(x) for x in re.search(r'^B = \{([^}]*)\}', out, re.M)[1].split(',')]
+    return rec
+
+
+def stream_candidates(n, sig, cap, end=400):
+    cfg = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_PROF_ORDER=1,
+               WZ_FH_ORBIT_CANON=1, WZ_FH_PROG_SEC=99999, WZ_FH_CELLSIZE=cap, WZ_FH_PROF_END=end)
+    with tempfile.TemporaryDirectory() as td:
+        dump = Path(td) / 'dump.txt'
+        run(n, sig, {**cfg, 'WZ_FH_DUMP': dump})
+        out = []
+        for line in dump.read_text().splitlines():
+            c, d = line.split('|')
+            out.append(([int(x) for x in c.split()], [int(x) for x in d.split()]))
+    return out
+
+
+def gen_complete(jobs):
+    tasks = []
+    six = {'ours-41': 'results/champions/champion_firsthit_bs42_41.txt',
+           'ours-42': 'results/champions/champion_firsthit_bs43_42.txt',
+           'ours-43': 'results/champions/champion_firsthit_bs44_43.txt',
+           'WZ-41': 'results/reference/wz_table1_bs42_41.txt',
+           'WZ-42': 'results/reference/wz_table1_bs43_42.txt',
+           'WZ-43': 'results/reference/wz_table1_bs44_43.txt'}
+    for name, f in six.items():
+        n, sig, (A, B, C, D) = read_solution(REFTREE / f)
+        asig = [abs(x) for x in sig]
+        for k in (0, 3):
+            tasks.append(('six:' + name, n, asig, C, D, 0, k))
+    # n = 44 production-class candidates (class (1,7,8,8)), budget 2e6 and small abort budgets
+    lines = (REFTREE / 'results/archive/n44_cands_20k.txt').read_text().splitlines()
+    picks = list(range(0, 24)) + list(range(1000, 20000, 1000))
+    for i in picks:
+        c, d = lines[i].split('|')
+        C, D = [int(x) for x in c.split()], [int(x) for x in d.split()]
+        tasks.append((f'n44:{i}', 44, [1, 7, 8, 8], C, D, 2000000, 3))
+    for i in range(0, 8):
+        c, d = lines[i].split('|')
+        C, D = [int(x) for x in c.split()], [int(x) for x in d.split()]
+        tasks.append((f'n44abort:{i}', 44, [1, 7, 8, 8], C, D, 977, 3))
+        tasks.append((f'n44k0:{i}', 44, [1, 7, 8, 8], C, D, 5000, 0))
+    # smaller-n stream candidates, exact and capped budgets
+    for n, sig, take in ((13, (2, 4, 3, 5), 40), (19, (0, 2, 5, 7), 40), (29, (0, 6, 1, 9), 24)):
+        cands = stream_candidates(n, sig, 4)
+        step = max(1, len(cands) // take)
+        for j, (C, D) in enumerate(cands[::step][:take]):
+            for budget in (0, 300):
+                tasks.append((f'n{n}:{j}', n, list(sig), C, D, budget, 3))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(complete_case, *t[1:]) for t in tasks]
+        recs = []
+        for t, f in zip(tasks, futs):
+            r = f.result()
+            r['name'] = t[0]
+            recs.append(r)
+    (HERE / 'complete_only.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'complete_only.json: {len(recs)} completions, verdicts '
+          f'{sorted(set((r["r"]) for r in recs))}')
+
+
+CANDLINE = re.compile(r'^CAND idx=(\d+) ci=(-?\d+) pi=(\d+) r=(\d) nodes=(\d+)', re.M)
+FHLINE = re.compile(r'^FIRSTHIT: idx=(\d+) profile_rank=(\d+) nodes_this_cand=(\d+) score=(\d+)', re.M)
+
+
+def firsthit_case(n, sig, extra):
+    cfg = {**dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_PROF_ORDER=1,
+                  WZ_FH_ORBIT_CANON=1, WZ_FH_PROG_SEC=99999, WZ_FH_CAND_LOG=1), **extra}
+    rc, out = run(n, sig, cfg)
+    rec = dict(n=n, sig=list(sig), settings={k: v for k, v in cfg.items() if k != 'WZ_FH_PROG_SEC'},
+               rc=rc, summary=summary(out),
+               cand=[[int(g) for g in m.groups()] for m in CANDLINE.finditer(out)])
+    m = FHLINE.search(out)
+    if m:
+        rec['firsthit'] = [int(g) for g in m.groups()]
+        for name in 'ABCD':
+            rec[name] = [int(x) for x in re.search(r'^' + name + r' = \{([^}]*)\}', out, re.M)[1].split(',')]
+    return rec
+
+
+def gen_firsthit(jobs):
+    tasks = []
+    for n in range(6, 14):
+        for sig in classes(n):
+            for budget in (0, 30):
+                tasks.append((n, sig, {'WZ_FH_AB_BUDGET': budget}))
+            tasks.append((n, sig, {'WZ_FH_AB_BUDGET': 40, 'WZ_FH_BUF_CAP': 8, 'WZ_FH_DRAIN_TOP': 3,
+                                   'WZ_FH_DRAIN_BATCHES': 2}))
+            tasks.append((n, sig, {'WZ_FH_AB_BUDGET': 40, 'WZ_FH_STREAM_REV': 1, 'WZ_FH_ORBIT_CANON': 0}))
+    for sig in classes(19):
+        tasks.append((19, sig, {'WZ_FH_AB_BUDGET': 500, 'WZ_FH_BUF_CAP': 64, 'WZ_FH_DRAIN_TOP': 8,
+                                'WZ_FH_MAX_CAND': 20000}))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: firsthit_case(*t), tasks))
+    (HERE / 'firsthit_runs.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'firsthit_runs.json: {len(recs)} runs, {sum(len(r["cand"]) for r in recs)} completions, '
+          f'{sum(1 for r in recs if "firsthit" in r)} hits')
+
+
+def gen_n29(jobs):
+    recs = []
+    for k in (3, 0):
+        cfg = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_FH_OUTER_K=k, WZ_FH_PROG_SEC=99999)
+        rc, out = run(29, (0, 6, 1, 9), cfg)
+        m = FHLINE.search(out)
+        rec = dict(n=29, sig=[0, 6, 1, 9], settings=cfg, rc=rc, summary=summary(out),
+                   firsthit=[int(g) for g in m.groups()] if m else None)
+        for name in 'ABCD':
+            mm = re.search(r'^' + name + r' = \{([^}]*)\}', out, re.M)
+            if mm:
+                rec[name] = [int(x) for x in mm[1].split(',')]
+        recs.append(rec)
+    (HERE / 'firsthit_n29.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print('firsthit_n29.json:', [r['firsthit'] for r in recs])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--only', default='')
+    ap.add_argument('--jobs', type=int, default=8)
+    a = ap.parse_args()
+    if not REF.exists():
+        sys.exit(f'{REF} missing: run make -C oracle ref (needs the reference tree)')
+    gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29)
+    for name, g in gens.items():
+        if not a.only or name in a.only.split(','):
+            g(a.jobs)
+
+
+if __name__ == '__main__':
+    main()

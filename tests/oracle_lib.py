@@ -1,0 +1,123 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: the oracle is the checker, never the thing measured or shipped.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parents[1]
+LIB = REPO / 'oracle' / '_build' / 'liboracle.so'
+
+
+class Config(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ('n', 'a', 'b', 'c', 'd', 'm6', 'thm211b', 'thm212', 'canon',
+                                        'orbit_q', 'endpos', 'order', 'rev', 'ab_prof', 'outer_k',
+                                        'no_canon_ab')] + [('abp_cap', C.c_longlong), ('no_cells', C.c_int)]
+
+
+class StreamStats(C.Structure):
+    _fields_ = [('leaves', C.c_longlong), ('hall_ok', C.c_longlong), ('dfs_nodes', C.c_longlong),
+                ('cand', C.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise FileNotFoundError(f'{LIB} missing: run make -C oracle')
+        L = C.CDLL(str(LIB))
+        L.wzo_create.restype = C.c_void_p
+        L.wzo_create.argtypes = [C.POINTER(Config)]
+        L.wzo_destroy.argtypes = [C.c_void_p]
+        L.wzo_num_cells.argtypes = [C.c_void_p]
+        L.wzo_cell_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.wzo_stream_cell.restype = C.c_longlong
+        L.wzo_stream_cell.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_void_p, C.c_longlong,
+                                      C.POINTER(StreamStats)]
+        L.wzo_complete.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_longlong,
+                                   C.POINTER(C.c_longlong), C.c_void_p, C.c_void_p]
+        L.wzo_flat_score.restype = C.c_longlong
+        L.wzo_flat_score.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        L.wzo_npaf_at.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.wzo_cell_front.restype = C.c_longlong
+        L.wzo_cell_front.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_int, C.c_longlong,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(StreamStats)]
+        L.wzo_outer_tables.argtypes = [C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _i8(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int8))
+
+
+class OracleProblem:
+    def __init__(self, n, sig, m6=1, thm211b=1, thm212=1, canon=0, orbit_q=0, endpos=0, order=1,
+                 rev=0, ab_prof=-1, outer_k=3, no_canon_ab=0, abp_cap=20000, no_cells=0):
+        self.n = n
+        self.m = 6 if m6 else 3
+        cfg = Config(n, *sig, m6, thm211b, thm212, canon, orbit_q, endpos, order, rev, ab_prof,
+                     outer_k, no_canon_ab, abp_cap, no_cells)
+        self.h = lib().wzo_create(C.byref(cfg))
+
+    def close(self):
+        if self.h:
+            lib().wzo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def num_cells(self):
+        return lib().wzo_num_cells(self.h)
+
+    def cell(self, pi):
+        px = (C.c_int * 8)()
+        py = (C.c_int * 8)()
+        kept, rows = C.c_int(), C.c_int()
+        lib().wzo_cell_info(self.h, pi, px, py, C.byref(kept), C.byref(rows))
+        return list(px[:self.m]), list(py[:self.m]), kept.value, rows.value
+
+    def stream(self, pi, cap=0, out_cap=0):
+        st = StreamStats()
+        out = np.zeros((max(out_cap, 1), 2, self.n), dtype=np.int8)
+        got = lib().wzo_stream_cell(self.h, pi, cap, out.ctypes.data, out_cap, C.byref(st))
+        return got, out[:min(got, out_cap)], st
+
+    def complete(self, C_, D_, budget, pi=-1):
+        n = self.n
+        A = np.zeros(n + 1, np.int8)
+        B = np.zeros(n + 1, np.int8)
+        nodes = C.c_longlong()
+        Cc, Dc = _i8(C_), _i8(D_)
+        r = lib().wzo_complete(self.h, pi, Cc.ctypes.data, Dc.ctypes.data, budget, C.byref(nodes),
+                               A.ctypes.data, B.ctypes.data)
+        return r, nodes.value, A, B
+
+    def front(self, pi, buf_cap, top_k, budget):
+        v = np.zeros(top_k, np.int32)
+        nd = np.zeros(top_k, np.int64)
+        idx = np.zeros(top_k, np.int64)
+        st = StreamStats()
+        got = lib().wzo_cell_front(self.h, pi, buf_cap, top_k, budget, v.ctypes.data, nd.ctypes.data,
+                                   idx.ctypes.data, C.byref(st))
+        return got, v, nd, idx, st
+
+
+def flat_score(n, C_, D_):
+    Cc, Dc = _i8(C_), _i8(D_)
+    return lib().wzo_flat_score(n, Cc.ctypes.data, Dc.ctypes.data)
+
+
+def dump_lines(cands):
+    """WZ_FH_DUMP text of candidates (wz_match.cpp:2743-2748)."""
+    out = []
+    for cd in cands:
+        out.append(''.join(f'{int(v)} ' for v in cd[0]) + '| ' + ''.join(f'{int(v)} ' for v in cd[1]) + '\n')
+    return out

@@ -1,0 +1,123 @@
+"""GPU parity: the HIP stream and completer against the reference's golden outputs.
+
+Bit-exact bar: per-cell candidate counts, leaf counts, hall_ok counts, DFS node counts and
+the SHA-1 of each cell's stream bytes (WZ_FH_DUMP format) must equal the reference's
+(tests/golden/stream_cells.json); completion verdicts, node counts (including the budget
+abort unwinding) and first A,B must equal WZ_FH_COMPLETE_C/_D's (complete_only.json).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import bs45_quantum_explorer_amd as wz
+from golden_data import complete_cases, stream_runs
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(rec):
+    return wz.Problem(rec['n'], rec['sig'], m6=1, thm211b=1, thm212=1, canon=rec['canon'], rev=rec['rev'],
+                      endpos=rec['endpos'], order=rec['order'])
+
+
+def _cells_of(rec, P, cand_cap=0, split_depth=0):
+    cells = [c[0] for c in rec['cells']]
+    if not cells:
+        return []
+    stats, offs, cd = P.stream(cells, cand_cap=cand_cap, split_depth=split_depth)
+    rows = []
+    for i, pi in enumerate(cells):
+        part = cd[offs[i]:offs[i + 1]]
+        h = hashlib.sha1(''.join(wz.dump_lines(rec['n'], part)).encode()).hexdigest()
+        st = stats[i]
+        rows.append([pi, st.cand, st.leaves, st.hall_ok, st.dfs_nodes, h])
+    return rows
+
+
+RUNS = stream_runs()
+SMALL = [r for r in RUNS if r['n'] <= 13]
+N19 = [r for r in RUNS if r['n'] == 19]
+
+
+@pytest.mark.parametrize('n', list(range(6, 14)))
+def test_stream_small_n_matches_reference(n):
+    runs = [r for r in SMALL if r['n'] == n]
+    assert runs
+    for rec in runs:
+        P = _problem(rec)
+        try:
+            live = P.live_cells()
+            assert live == [c[0] for c in rec['cells']], 'cell list differs'
+            got = _cells_of(rec, P)
+            assert got == rec['cells'], (rec['n'], rec['sig'], rec['canon'], rec['rev'], rec['endpos'])
+        finally:
+            P.close()
+
+
+@pytest.mark.parametrize('split', [1, 2, 4])
+def test_stream_split_depth_invariance(split):
+    rec = next(r for r in SMALL if r['n'] == 13 and r['canon'] == 0 and r['rev'] == 1)
+    P = _problem(rec)
+    try:
+        assert _cells_of(rec, P, split_depth=split) == rec['cells']
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize('idx', range(len(N19)))
+def test_stream_n19_matches_reference(idx):
+    rec = N19[idx]
+    P = _problem(rec)
+    try:
+        assert _cells_of(rec, P) == rec['cells']
+    finally:
+        P.close()
+
+
+def test_stream_candidate_cap_is_exact_prefix():
+    """WZ_FH_CELLSIZE semantics: a capped cell streams exactly the first `cap` candidates."""
+    rec = next(r for r in N19 if r['canon'] == 1 and r['rev'] == 0)
+    P = _problem(rec)
+    try:
+        cells = [c[0] for c in rec['cells'] if c[1] >= 5][:40]
+        full_stats, full_offs, full_cd = P.stream(cells)
+        for cap in (1, 3, 17):
+            st, offs, cd = P.stream(cells, cand_cap=cap)
+            for i in range(len(cells)):
+                want = full_cd[full_offs[i]:full_offs[i] + min(cap, full_stats[i].cand)]
+                assert np.array_equal(cd[offs[i]:offs[i + 1]], want)
+                assert st[i].cand == min(cap, full_stats[i].cand)
+                assert st[i].capped == (1 if full_stats[i].cand >= cap else 0)
+                assert st[i].dfs_nodes <= full_stats[i].dfs_nodes
+    finally:
+        P.close()
+
+
+CASES = complete_cases()
+
+
+def _complete_group(cases):
+    n, sig, k = cases[0]['n'], cases[0]['sig'], cases[0]['outer_k']
+    P = wz.Problem(n, sig, m6=0, thm211b=0, thm212=0, outer_k=k)
+    try:
+        for budget in sorted(set(c['budget'] for c in cases)):
+            grp = [c for c in cases if c['budget'] == budget]
+            cd = np.array([[wz.pack(c['C']), wz.pack(c['D'])] for c in grp], dtype=np.uint64)
+            out = P.complete(cd, budget=budget)
+            for c, o in zip(grp, out):
+                assert (o.verdict, o.nodes) == (c['r'], c['nodes']), (c['name'], budget, k)
+                if c['r'] == 0:
+                    assert list(wz.unpack(o.A, n + 1)) == c['A'] and list(wz.unpack(o.B, n + 1)) == c['B']
+    finally:
+        P.close()
+
+
+GROUPS = {}
+for c in CASES:
+    GROUPS.setdefault((c['n'], tuple(c['sig']), c['outer_k']), []).append(c)
+
+
+@pytest.mark.parametrize('key', sorted(GROUPS), ids=lambda k: f'n{k[0]}-{k[1]}-k{k[2]}')
+def test_complete_only_matches_reference(key):
+    _complete_group(GROUPS[key])
