@@ -18,7 +18,7 @@ OK, EINVAL, ENOMEM, EDEVICE, ENODEV, ECAPACITY, EINTERNAL, ETRUNC = 0, -1, -2, -
 HIT, PREREJECT, CLEAN_NO, ABORTED, INTERNAL = 0, 1, 2, 3, 4
 
 EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy', 'wzfh_last_error',
-           'wzfh_num_cells', 'wzfh_cell_info', 'wzfh_order_digest', 'wzfh_stream_cells', 'wzfh_complete',
+           'wzfh_num_cells', 'wzfh_cell_info', 'wzfh_live_cells', 'wzfh_order_digest', 'wzfh_stream_cells', 'wzfh_complete',
            'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms')
 
 
@@ -79,6 +79,7 @@ def lib():
         L.wzfh_num_cells.argtypes = [vp]
         L.wzfh_cell_info.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.wzfh_order_digest.argtypes = [vp, C.c_char_p, C.c_int]
+        L.wzfh_live_cells.argtypes = [vp, vp, C.c_int]
         L.wzfh_stream_cells.argtypes = [vp, vp, C.c_int, C.POINTER(StreamOpts), vp, vp, C.c_longlong, vp]
         L.wzfh_complete.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
         L.wzfh_search_cells.argtypes = [vp, vp, C.c_int, C.POINTER(SearchOpts), vp]
@@ -150,12 +151,10 @@ class Problem:
 
     def live_cells(self):
         """Cells the arm loop streams: kept by orbit canon and not proven dead."""
-        out = []
-        for pi in range(self.num_cells()):
-            _, _, kept, rows = self.cell(pi)
-            if kept and rows != -1:
-                out.append(pi)
-        return out
+        cnt = lib().wzfh_live_cells(self.h, None, 0)
+        out = np.zeros(max(cnt, 1), dtype=np.int32)
+        lib().wzfh_live_cells(self.h, out.ctypes.data, cnt)
+        return [int(x) for x in out[:cnt]]
 
     def digest(self):
         buf = C.create_string_buffer(64)

@@ -68,6 +68,14 @@ struct DeviceState {
   DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
+  // the last stream result, reused when the caller asks again for the same cells and
+  // options (count first, then fetch) so the kernel runs once
+  std::vector<int> memo_cells;
+  wzfh_stream_opts memo_opts{};
+  std::vector<wzfh_cell_stats> memo_stats;
+  std::vector<uint64_t> memo_cd;
+  std::vector<long long> memo_offs;
+  bool memo_valid = false;
 };
 
 static int quad_bits(const int *v) {   // sign bits x1,y1,x2,y2 -> bit3..bit0 (set = -1)
@@ -541,6 +549,18 @@ int wzfh_cell_info(const wzfh_problem *p, int pi, int *px, int *py, int *kept, i
   return WZFH_OK;
 }
 
+int wzfh_live_cells(const wzfh_problem *p, int *out, int cap) {
+  if (!p) return WZFH_EINVAL;
+  int cnt = 0;
+  for (size_t i = 0; i < p->cells.size(); i++) {
+    if (!p->kept[i]) continue;
+    if (p->ab_prof && p->cell_rows[i] < 0) continue;
+    if (out && cnt < cap) out[cnt] = (int)i;
+    cnt++;
+  }
+  return cnt;
+}
+
 int wzfh_order_digest(const wzfh_problem *p, char *buf, int buflen) {
   if (!p || !buf || buflen < 34) return WZFH_EINVAL;
   std::snprintf(buf, buflen, "%s", p->digest.c_str());
@@ -555,14 +575,29 @@ int wzfh_stream_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
   wzfh_stream_opts o{};
   if (opts) o = *opts;
-  StreamOut so;
-  int rc = run_stream(*p, cells, ncells, o, stats, so);
-  if (rc != WZFH_OK) return rc;
-  for (int i = 0; i <= ncells; i++) offsets[i] = so.cell_begin[i];
-  const long long total = so.cell_begin[ncells];
+  DeviceState *S = p->dev;
+  const bool same = S->memo_valid && S->memo_cells.size() == (size_t)ncells &&
+                    std::equal(cells, cells + ncells, S->memo_cells.begin()) &&
+                    std::memcmp(&S->memo_opts, &o, sizeof o) == 0;
+  if (!same) {
+    StreamOut so;
+    S->memo_valid = false;
+    S->memo_stats.assign(ncells, wzfh_cell_stats{});
+    int rc = run_stream(*p, cells, ncells, o, S->memo_stats.data(), so);
+    if (rc != WZFH_OK) return rc;
+    S->memo_cells.assign(cells, cells + ncells);
+    S->memo_opts = o;
+    S->memo_offs = so.cell_begin;
+    S->memo_cd.resize(2 * so.cands.size());
+    for (size_t j = 0; j < so.cands.size(); j++) { S->memo_cd[2 * j] = so.cands[j].X; S->memo_cd[2 * j + 1] = so.cands[j].Y; }
+    S->memo_valid = true;
+  }
+  std::copy(S->memo_stats.begin(), S->memo_stats.end(), stats);
+  for (int i = 0; i <= ncells; i++) offsets[i] = S->memo_offs[i];
+  const long long total = S->memo_offs[ncells];
   if (cd) {
     if (total > cd_capacity) return WZFH_ECAPACITY;
-    for (long long j = 0; j < total; j++) { cd[2 * j] = so.cands[j].X; cd[2 * j + 1] = so.cands[j].Y; }
+    std::copy(S->memo_cd.begin(), S->memo_cd.begin() + 2 * total, cd);
   }
   return WZFH_OK;
 }

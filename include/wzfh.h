@@ -85,6 +85,9 @@ int wzfh_num_cells(const wzfh_problem *p);
 /* Cell pi of the ordered list: px, py (m ints each), kept (orbit-canon keep flag),
  * rows (A,B profile rows of the cell; -1 = proven dead, -2 = rows off). */
 int wzfh_cell_info(const wzfh_problem *p, int pi, int *px, int *py, int *kept, int *rows);
+/* Indices of the cells an arm streams (kept by orbit canon and not proven dead), in list
+ * order, written to out[0..cap); returns the count (the cell loop's skips, wz_match.cpp:2499-2521). */
+int wzfh_live_cells(const wzfh_problem *p, int *out, int cap);
 /* 64-bit FNV digests "<list>:<kept>" of the ordered cell list under order 3
  * (wz_match.cpp:1834-1856); empty string otherwise. buf needs 34 bytes. */
 int wzfh_order_digest(const wzfh_problem *p, char *buf, int buflen);

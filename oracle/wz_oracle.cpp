@@ -413,7 +413,7 @@ struct Stream {
   int L, m, half;
   int tx[8], ty[8], tot[8], px[8] = {0}, py[8] = {0}, placed[8] = {0};
   int X[64], Y[64];
-  long long cap, nout = 0, leaves = 0, ok = 0, nodes = 0;
+  long long cap, nout = 0, leaves = 0, ok = 0, nodes = 0, node_budget = 0;
   bool stop = false;
   int8_t *out; long long out_cap;
   Stream(const Problem &p, const Cell &cell, long long cap_, int8_t *o, long long oc)
@@ -452,6 +452,7 @@ struct Stream {
   }
   void rec(int d) {
     if (stop) return;
+    if (node_budget > 0 && nodes >= node_budget) { stop = true; return; }   // node index >= budget
     nodes++;
     for (int c = 0; c < m; c++) {
       int rem = tot[c] - placed[c], dx = tx[c] - px[c], dy = ty[c] - py[c];
@@ -694,6 +695,23 @@ void *wzo_create(const wzo_config *cfg) {
   return P;
 }
 
+void *wzo_create_with_cells(const wzo_config *cfg, const int8_t *pxpy, int ncells) {
+  wzo_config c2 = *cfg;
+  c2.no_cells = 1;
+  Problem *P = (Problem *)wzo_create(&c2);
+  P->cfg.no_cells = 0;
+  P->ab_prof = cfg->ab_prof < 0 ? (cfg->m6 != 0) : (cfg->ab_prof != 0);
+  for (int i = 0; i < ncells; i++) {
+    Cell c;
+    c.px.assign(pxpy + 16 * i, pxpy + 16 * i + P->m);
+    c.py.assign(pxpy + 16 * i + 8, pxpy + 16 * i + 8 + P->m);
+    P->cells.push_back(c);
+  }
+  P->kept.assign(P->cells.size(), 1);
+  if (P->ab_prof) build_rows(*P);
+  return P;
+}
+
 void wzo_destroy(void *h) { delete (Problem *)h; }
 
 int wzo_num_cells(void *h) { return (int)((Problem *)h)->cells.size(); }
@@ -706,6 +724,16 @@ int wzo_cell_info(void *h, int pi, int *px, int *py, int *kept, int *rows) {
   if (!P.ab_prof) *rows = -2;
   else *rows = P.rowkey[pi] < 0 ? -1 : (int)P.rowlists[P.rowkey[pi]].size();
   return 0;
+}
+
+long long wzo_stream_cell_budget(void *h, int pi, long long cap, long long node_budget, int8_t *out,
+                                 long long out_cap, wzo_stream_stats *st) {
+  Problem &P = *(Problem *)h;
+  Stream S(P, P.cells[pi], cap, out, out_cap);
+  S.node_budget = node_budget;
+  S.rec(0);
+  if (st) { st->leaves = S.leaves; st->hall_ok = S.ok; st->dfs_nodes = S.nodes; st->cand = S.nout; }
+  return S.nout;
 }
 
 long long wzo_stream_cell(void *h, int pi, long long cap, int8_t *out, long long out_cap,
@@ -750,10 +778,16 @@ int wzo_npaf_at(const int8_t *A, const int8_t *B, int n1, const int8_t *C, const
 long long wzo_cell_front(void *h, int pi, long long buf_cap, int top_k, long long budget,
                          int *verdicts, long long *nodes, long long *stream_idx,
                          wzo_stream_stats *st) {
+  return wzo_cell_front_budget(h, pi, buf_cap, 0, top_k, budget, verdicts, nodes, stream_idx, st);
+}
+
+long long wzo_cell_front_budget(void *h, int pi, long long buf_cap, long long node_budget, int top_k,
+                                long long budget, int *verdicts, long long *nodes, long long *stream_idx,
+                                wzo_stream_stats *st) {
   Problem &P = *(Problem *)h;
   int n = P.n;
   std::vector<int8_t> buf((size_t)buf_cap * 2 * n);
-  long long got = wzo_stream_cell(h, pi, buf_cap, buf.data(), buf_cap, st);
+  long long got = wzo_stream_cell_budget(h, pi, buf_cap, node_budget, buf.data(), buf_cap, st);
   std::vector<std::pair<long long, long long>> key(got);
   for (long long i = 0; i < got; i++) key[i] = {wzo_flat_score(n, &buf[i * 2 * n], &buf[i * 2 * n + n]), i};
   std::stable_sort(key.begin(), key.end(), [](auto &x, auto &y) { return x.first < y.first; });

@@ -46,6 +46,9 @@ typedef struct {
 } wzo_stream_stats;
 
 void *wzo_create(const wzo_config *cfg);
+/* A problem over an explicit cell list (px[8], py[8] as int8 per cell), with the A,B
+ * profile rows built for exactly those cells (a sample of a larger class list). */
+void *wzo_create_with_cells(const wzo_config *cfg, const int8_t *pxpy, int ncells);
 void wzo_destroy(void *h);
 int wzo_num_cells(void *h);
 /* px, py: m ints each. kept: orbit-canon keep flag (1 when canon is off).
@@ -71,6 +74,13 @@ int wzo_npaf_at(const int8_t *A, const int8_t *B, int n1, const int8_t *C, const
 long long wzo_cell_front(void *h, int pi, long long buf_cap, int top_k, long long budget,
                          int *verdicts, long long *nodes, long long *stream_idx,
                          wzo_stream_stats *st);
+/* Same with a DFS node budget: only nodes with DFS index < node_budget are visited
+ * (the serving policy's per-cell budget; the reference has no such cap). */
+long long wzo_stream_cell_budget(void *h, int pi, long long cap, long long node_budget, int8_t *out,
+                                 long long out_cap, wzo_stream_stats *st);
+long long wzo_cell_front_budget(void *h, int pi, long long buf_cap, long long node_budget, int top_k,
+                                long long budget, int *verdicts, long long *nodes, long long *stream_idx,
+                                wzo_stream_stats *st);
 /* A1 outer tables (wz_match.cpp:560-598) for test cross-checks. */
 void wzo_outer_tables(uint8_t *outer2 /*16*5*9*/, uint8_t *outer3 /*128*5*9*13*/);
 

@@ -19,6 +19,8 @@ Fixtures:
                       with WZ_FH_CAND_LOG: FIRSTHIT line, solution, summary counters
                       and the per-completion (idx, ci, pi, r, nodes) sequence
   firsthit_n29.json   the n = 29 known answer (WZ_FH_M6=1, default completer)
+  solutions.json      the banked solution files (results/champions, results/reference)
+                      and the quarantined invalid one (negative control)
 
 Usage: python tests/golden/make_golden.py [--only NAME] [--jobs 8]
 """
@@ -261,6 +263,24 @@ def gen_n29(jobs):
     print('firsthit_n29.json:', [r['firsthit'] for r in recs])
 
 
+def gen_solutions(jobs):
+    """The reference's banked solution files (data: n, signature, A, B, C, D), plus the
+    quarantined known-invalid one as a negative control."""
+    recs = []
+    for sub in ('results/champions', 'results/reference', 'results/quarantine'):
+        for f in sorted((REFTREE / sub).glob('*.txt')):
+            rows = [l.split() for l in f.read_text().splitlines() if l.strip() and not l.startswith('#')]
+            try:
+                n = int(rows[0][0])
+                seqs = [[int(x) for x in r] for r in rows[2:6]]
+            except (ValueError, IndexError):
+                continue
+            recs.append(dict(file=f'{sub}/{f.name}', n=n, sig=[int(x) for x in rows[1][:4]], A=seqs[0], B=seqs[1],
+                             C=seqs[2], D=seqs[3], quarantined=sub.endswith('quarantine')))
+    (HERE / 'solutions.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'solutions.json: {len(recs)} files')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--only', default='')
@@ -268,7 +288,8 @@ def main():
     a = ap.parse_args()
     if not REF.exists():
         sys.exit(f'{REF} missing: run make -C oracle ref (needs the reference tree)')
-    gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29)
+    gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29,
+                solutions=gen_solutions)
     for name, g in gens.items():
         if not a.only or name in a.only.split(','):
             g(a.jobs)

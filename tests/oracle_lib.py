@@ -34,6 +34,8 @@ def lib():
         L.wzo_create.restype = C.c_void_p
         L.wzo_create.argtypes = [C.POINTER(Config)]
         L.wzo_destroy.argtypes = [C.c_void_p]
+        L.wzo_create_with_cells.restype = C.c_void_p
+        L.wzo_create_with_cells.argtypes = [C.POINTER(Config), C.c_void_p, C.c_int]
         L.wzo_num_cells.argtypes = [C.c_void_p]
         L.wzo_cell_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -49,6 +51,13 @@ def lib():
         L.wzo_cell_front.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_int, C.c_longlong,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(StreamStats)]
         L.wzo_outer_tables.argtypes = [C.c_void_p, C.c_void_p]
+        L.wzo_stream_cell_budget.restype = C.c_longlong
+        L.wzo_stream_cell_budget.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_void_p,
+                                             C.c_longlong, C.POINTER(StreamStats)]
+        L.wzo_cell_front_budget.restype = C.c_longlong
+        L.wzo_cell_front_budget.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_int,
+                                            C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.POINTER(StreamStats)]
         _lib = L
     return _lib
 
@@ -59,12 +68,19 @@ def _i8(a):
 
 class OracleProblem:
     def __init__(self, n, sig, m6=1, thm211b=1, thm212=1, canon=0, orbit_q=0, endpos=0, order=1,
-                 rev=0, ab_prof=-1, outer_k=3, no_canon_ab=0, abp_cap=20000, no_cells=0):
+                 rev=0, ab_prof=-1, outer_k=3, no_canon_ab=0, abp_cap=20000, no_cells=0, cells=None):
         self.n = n
         self.m = 6 if m6 else 3
         cfg = Config(n, *sig, m6, thm211b, thm212, canon, orbit_q, endpos, order, rev, ab_prof,
                      outer_k, no_canon_ab, abp_cap, no_cells)
-        self.h = lib().wzo_create(C.byref(cfg))
+        if cells is None:
+            self.h = lib().wzo_create(C.byref(cfg))
+        else:   # explicit (px, py) list: cell i of this problem is cells[i]
+            arr = np.zeros((len(cells), 16), dtype=np.int8)
+            for i, (px, py) in enumerate(cells):
+                arr[i, :len(px)] = px
+                arr[i, 8:8 + len(py)] = py
+            self.h = lib().wzo_create_with_cells(C.byref(cfg), arr.ctypes.data, len(cells))
 
     def close(self):
         if self.h:
@@ -89,6 +105,21 @@ class OracleProblem:
         out = np.zeros((max(out_cap, 1), 2, self.n), dtype=np.int8)
         got = lib().wzo_stream_cell(self.h, pi, cap, out.ctypes.data, out_cap, C.byref(st))
         return got, out[:min(got, out_cap)], st
+
+    def stream_budget(self, pi, cap, node_budget, out_cap):
+        st = StreamStats()
+        out = np.zeros((max(out_cap, 1), 2, self.n), dtype=np.int8)
+        got = lib().wzo_stream_cell_budget(self.h, pi, cap, node_budget, out.ctypes.data, out_cap, C.byref(st))
+        return got, out[:min(got, out_cap)], st
+
+    def front_budget(self, pi, buf_cap, node_budget, top_k, budget):
+        v = np.zeros(top_k, np.int32)
+        nd = np.zeros(top_k, np.int64)
+        idx = np.zeros(top_k, np.int64)
+        st = StreamStats()
+        got = lib().wzo_cell_front_budget(self.h, pi, buf_cap, node_budget, top_k, budget, v.ctypes.data,
+                                          nd.ctypes.data, idx.ctypes.data, C.byref(st))
+        return got, v, nd, idx, st
 
     def complete(self, C_, D_, budget, pi=-1):
         n = self.n

@@ -1,0 +1,28 @@
+"""Host setup (the product's per-(n, class) cell list, orbit canon, order, dead-cell rows)
+against the reference's cell lists (tests/golden/stream_cells.json)."""
+import pytest
+
+import bs45_quantum_explorer_amd as wz
+from golden_data import stream_runs
+
+RUNS = stream_runs()
+
+
+@pytest.mark.parametrize('n', sorted(set(r['n'] for r in RUNS)))
+def test_live_cell_list_matches_reference(n):
+    for rec in (r for r in RUNS if r['n'] == n):
+        P = wz.Problem(n, rec['sig'], device=False, m6=1, thm211b=1, thm212=1, canon=rec['canon'], rev=rec['rev'],
+                       endpos=rec['endpos'], order=rec['order'])
+        try:
+            assert P.num_cells() == rec['profiles']
+            assert P.live_cells() == [c[0] for c in rec['cells']]
+        finally:
+            P.close()
+
+
+def test_n44_class_list_size_matches_reference_probe():
+    # the reference's n=44 (1,7,8,8) setup reports 977,601 mod-6 cells
+    P = wz.Problem(44, (1, 7, 8, 8), device=False)
+    assert P.num_cells() == 977601
+    px, py, kept, rows = P.cell(0)
+    assert sum(px) == 8 and sum(py) == 8
