@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
       // uniform DFS state
       int d = 0, k = -1, state = S_ENTER;
       int sumA = 0, sumB = 0;
-      uint64_t PA = 0, PB = 0;    // 8-bit class fields of the A,B class sums, value + 64
+      uint64_t PA = 0x4040404040404040ull, PB = PA;   // 8-bit class fields: class sum + 64
       long long cur = 0;
       int dab = 0;                // lane s: Dab[s] of the current state
       int st_k = 0, st_omask = 0, st_ties = 0, st_sums = 0;   // lane l = level l records

@@ -7,6 +7,7 @@ tools/test_outer_tables.py), which diff these same lines."""
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -23,11 +24,23 @@ KEYS = ('candidates_streamed', 'pre_filter_rejected', 'backtracks_entered', 'cle
 ENV0 = {k: v for k, v in os.environ.items() if not k.startswith(('WZ_', 'FH_'))}
 
 
-def run_cli(n, sig, settings, timeout=300):
+def run_cli(n, sig, settings, timeout=300, echo_progress=False):
     env = {**ENV0, **{k: str(v) for k, v in settings.items()}}
-    p = subprocess.run([str(wz.CLIPATH), str(n), *map(str, sig)], env=env, text=True, capture_output=True,
-                       timeout=timeout)
-    return p.returncode, p.stdout
+    if not echo_progress:
+        p = subprocess.run([str(wz.CLIPATH), str(n), *map(str, sig)], env=env, text=True, capture_output=True,
+                           timeout=timeout)
+        return p.returncode, p.stdout
+    # long runs: pass the arm's own progress lines through as they are printed
+    env['WZ_FH_PROG_SEC'] = '20'
+    p = subprocess.Popen([str(wz.CLIPATH), str(n), *map(str, sig)], env=env, text=True, stdout=subprocess.PIPE)
+    lines = []
+    for line in p.stdout:
+        lines.append(line)
+        if line.startswith('[firsthit progress]'):
+            sys.__stderr__.write(line)
+            sys.__stderr__.flush()
+    p.wait(timeout=timeout)
+    return p.returncode, ''.join(lines)
 
 
 def summary(out):
@@ -76,7 +89,7 @@ def test_cli_cellsize_matches_reference_stream():
 def test_cli_n29_known_answer():
     """The reference's n = 29 first hit (WZ_FH_M6=1, default completer): idx, cell, nodes."""
     for rec in load('firsthit_n29.json'):
-        rc, out = run_cli(29, rec['sig'], rec['settings'], timeout=600)
+        rc, out = run_cli(29, rec['sig'], rec['settings'], timeout=600, echo_progress=True)
         assert rc == rec['rc'] == 0
         m = FH.search(out)
         assert [int(g) for g in m.groups()] == rec['firsthit']
