@@ -74,6 +74,10 @@ def sample_cells(live, k, seed):
 _PROBS = []
 
 
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
 def percentile(v, q):
     return float(np.percentile(np.asarray(v, dtype=np.float64), q)) if len(v) else 0.0
 
@@ -111,6 +115,7 @@ def main():
         cells.append(sample_cells(live, a.cells_per_class, a.seed * 1000 + i))
     setup_s = time.time() - t_setup
     total_cells = sum(len(c) for c in cells)
+    log(f'setup: {len(cls)} classes, {total_cells} cells, {setup_s:.1f} s')
 
     def step_batched():
         res = []
@@ -126,8 +131,9 @@ def main():
             res.append(r)
         return res, st_ms, cp_ms, st_units
 
-    for _ in range(a.warmup):
+    for w in range(a.warmup):
         step_batched()
+        log(f'warmup step {w + 1}/{a.warmup} done')
     times, kern = [], []
     results = None
     for _ in range(a.steps):
@@ -135,6 +141,8 @@ def main():
         results, st_ms, cp_ms, st_units = step_batched()
         times.append(time.perf_counter() - t0)
         kern.append((st_ms, cp_ms, st_units))
+        log(f'step {len(times)}/{a.steps}: {times[-1] * 1e3:.1f} ms (stream kernels {st_ms:.1f} ms, '
+            f'completer {cp_ms:.1f} ms)')
     ms_per_step = 1000.0 * float(np.mean(times))
     cells_per_s = total_cells / float(np.mean(times))
     streamed = sum(r.stream.cand for rr in results for r in rr)
@@ -155,6 +163,7 @@ def main():
             plain_lat.append(time.perf_counter() - t1)
             plain_res[(ci, pi)] = r[0]
             nplain += 1
+        log(f'plain: class {ci + 1}/{len(probs)} done ({nplain} cells)')
     plain_s = time.perf_counter() - t0
     plain_cells_per_s = nplain / plain_s if plain_s > 0 else 0.0
     # the batched and plain paths must agree exactly on every cell they share
@@ -245,7 +254,11 @@ def cpu_baseline(a, cls, cells):
         return got
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
-        cands = sum(ex.map(one, work))
+        cands = 0
+        for k, c in enumerate(ex.map(one, work)):
+            cands += c
+            if (k + 1) % threads == 0:
+                log(f'cpu baseline: {k + 1}/{len(work)} cells')
     dt = time.perf_counter() - t0
     return {'value': round(len(work) / dt, 4), 'unit': 'cells/s', 'cores': threads, 'kind': 'port',
             'sample': f'{len(work)} cells ({a.cpu_cells} per class) of the same workload, same unit',

@@ -21,6 +21,9 @@
 namespace wzfh {
 
 hipError_t launch_stream(const StreamArgs &a, int blocks, hipStream_t st);
+hipError_t launch_stream_finalize(const StreamArgs &a, uint64_t *noff, uint64_t *loff, uint64_t *cell_tot,
+                                  unsigned long long ncands, KeptCand *kept, unsigned long long *nkept,
+                                  hipStream_t st);
 hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, hipStream_t st);
 
 static thread_local std::string g_err;
@@ -65,9 +68,12 @@ struct DeviceState {
   std::vector<uint8_t> kab_host;
   // per-call buffers
   DBuf cell_res, cell_rootok, cut, item_nodes, item_leaves, item_cands, cands, ctr;
+  DBuf item_done, cell_lock, cell_pp, cell_pn, cell_pc, noff, loff, cell_tot, kept;
   DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
+  DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
+  unsigned long long last_telem[4] = {0, 0, 0, 0};
   // the last stream result, reused when the caller asks again for the same cells and
   // options (count first, then fetch) so the kernel runs once
   std::vector<int> memo_cells;
@@ -226,9 +232,10 @@ int device_init(wzfh_problem &P) {
 void device_free(wzfh_problem &P) {
   DeviceState *S = P.dev;
   if (!S) return;
-  for (DBuf *b : {&S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->posidx, &S->rows, &S->kab,
+  for (DBuf *b : {&S->telem, &S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->posidx, &S->rows, &S->kab,
                   &S->cell_res, &S->cell_rootok, &S->cut, &S->item_nodes, &S->item_leaves, &S->item_cands,
-                  &S->cands, &S->ctr, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
+                  &S->cands, &S->ctr, &S->item_done, &S->cell_lock, &S->cell_pp, &S->cell_pn, &S->cell_pc,
+                  &S->noff, &S->loff, &S->cell_tot, &S->kept, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
                   &S->c_B, &S->c_scratch})
     b->release();
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
@@ -248,8 +255,10 @@ static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncel
   const int half = P.n / 2;
   int d0 = o.split_depth;
   if (d0 <= 0) {
+    // capped cells are DFS prefixes: split deep so a prefix spans many subtrees; whole
+    // cells: enough paths per cell to fill the GPU without huge per-item arrays
     const bool capped = o.cand_cap > 0 || o.node_budget > 0;
-    if (capped) d0 = 3;
+    if (capped) d0 = P.n >= 30 ? 6 : 4;
     else d0 = ncells >= 4096 ? 2 : (ncells >= 256 ? 3 : 4);
   }
   return std::max(1, std::min(d0, std::max(1, half)));
@@ -261,7 +270,7 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
   StreamArgs a = S->base;
   const int d0 = auto_split(P, o, ncells);
   const int npaths = a.nch0 * (1 << (3 * (d0 - 1)));
-  int window = o.window > 0 ? o.window : ((o.cand_cap > 0 || o.node_budget > 0) ? std::min(npaths, 8) : npaths);
+  int window = o.window > 0 ? o.window : ((o.cand_cap > 0 || o.node_budget > 0) ? std::min(npaths, 64) : npaths);
   window = std::max(1, std::min(window, npaths));
   a.d0 = d0; a.npaths = npaths; a.window = window; a.ncells = ncells;
   a.node_budget = o.node_budget; a.cand_cap = o.cand_cap;
@@ -284,12 +293,17 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
   const size_t nitems = (size_t)ncells * npaths;
   if (S->cell_res.ensure(res.size() * 8) || S->cell_rootok.ensure(ncells * 4) || S->cut.ensure(ncells * 4) ||
       S->item_nodes.ensure(nitems * 8) || S->item_leaves.ensure(nitems * 4) || S->item_cands.ensure(nitems * 4) ||
-      S->ctr.ensure(64))
+      S->item_done.ensure(nitems * 4) || S->cell_lock.ensure(ncells * 4) || S->cell_pp.ensure(ncells * 4) ||
+      S->cell_pn.ensure(ncells * 8) || S->cell_pc.ensure(ncells * 8) || S->noff.ensure(nitems * 8) ||
+      S->loff.ensure(nitems * 8) || S->cell_tot.ensure(ncells * 24) || S->ctr.ensure(64))
     return WZFH_ENOMEM;
-  unsigned long long cap = o.cand_cap > 0 ? (unsigned long long)ncells * (unsigned long long)(o.cand_cap + 512)
-                                          : (unsigned long long)std::max<size_t>(1 << 20, S->cands.bytes / sizeof(CandRec));
+  // candidate buffer: what the caps allow, at most 4M records up front, grown on overflow
+  unsigned long long want = o.cand_cap > 0 ? (unsigned long long)ncells * (unsigned long long)(o.cand_cap + 512)
+                                           : (1ull << 20);
+  unsigned long long cap = std::max<unsigned long long>(std::min<unsigned long long>(want, 1ull << 22),
+                                                        S->cands.bytes / sizeof(CandRec));
   for (int attempt = 0; attempt < 8; attempt++) {
-    if (S->cands.ensure(cap * sizeof(CandRec))) return WZFH_ENOMEM;
+    if (S->cands.ensure(cap * sizeof(CandRec)) || S->kept.ensure(cap * sizeof(KeptCand))) return WZFH_ENOMEM;
     WZ_HIP(hipMemcpyAsync(S->cell_res.p, res.data(), res.size() * 8, hipMemcpyHostToDevice, S->stream));
     WZ_HIP(hipMemcpyAsync(S->cell_rootok.p, rootok.data(), ncells * 4, hipMemcpyHostToDevice, S->stream));
     WZ_HIP(hipMemcpyAsync(S->order.p, S->angle_order.data(), 200, hipMemcpyHostToDevice, S->stream));
@@ -297,6 +311,11 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     WZ_HIP(hipMemsetAsync(S->item_nodes.p, 0, nitems * 8, S->stream));
     WZ_HIP(hipMemsetAsync(S->item_leaves.p, 0, nitems * 4, S->stream));
     WZ_HIP(hipMemsetAsync(S->item_cands.p, 0, nitems * 4, S->stream));
+    WZ_HIP(hipMemsetAsync(S->item_done.p, 0, nitems * 4, S->stream));
+    WZ_HIP(hipMemsetAsync(S->cell_lock.p, 0, ncells * 4, S->stream));
+    WZ_HIP(hipMemsetAsync(S->cell_pp.p, 0, ncells * 4, S->stream));
+    WZ_HIP(hipMemsetAsync(S->cell_pn.p, 0, ncells * 8, S->stream));
+    WZ_HIP(hipMemsetAsync(S->cell_pc.p, 0, ncells * 8, S->stream));
     WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
     WZ_HIP(hipMemsetAsync(S->rej.p, 0, 200 * sizeof(unsigned), S->stream));
     a.cell_res = S->cell_res.as<uint64_t>();
@@ -309,9 +328,20 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     a.item_nodes = S->item_nodes.as<uint64_t>();
     a.item_leaves = S->item_leaves.as<uint32_t>();
     a.item_cands = S->item_cands.as<uint32_t>();
+    a.item_done = S->item_done.as<uint32_t>();
+    a.cell_lock = S->cell_lock.as<int>();
+    a.cell_pp = S->cell_pp.as<int>();
+    a.cell_pn = S->cell_pn.as<unsigned long long>();
+    a.cell_pc = S->cell_pc.as<unsigned long long>();
     a.cands = S->cands.as<CandRec>();
     a.cand_capacity = cap;
     a.angle_rej = S->rej.as<unsigned>();
+    a.telem = nullptr;
+    if (getenv("WZFH_TELEMETRY") && atoi(getenv("WZFH_TELEMETRY"))) {
+      if (S->telem.ensure(64)) return WZFH_ENOMEM;
+      WZ_HIP(hipMemsetAsync(S->telem.p, 0, 64, S->stream));
+      a.telem = S->telem.as<unsigned long long>();
+    }
     const unsigned long long work_items = (unsigned long long)((npaths + window - 1) / window) * ncells * window;
     int blocks = (int)std::min<unsigned long long>((unsigned long long)S->num_cu * 8, (work_items + 255) / 256 + 1);
     blocks = std::max(blocks, 1);
@@ -324,28 +354,31 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     float ms = 0;
     WZ_HIP(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
     S->last_ms[0] = ms;
+    if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 32, hipMemcpyDeviceToHost));
     if (ctr[1] > cap) { cap = ctr[1] + (ctr[1] >> 2) + 1024; continue; }
-    // ---- copy back and order
-    std::vector<uint64_t> inodes(nitems);
-    std::vector<uint32_t> ileaves(nitems), icands(nitems);
-    std::vector<CandRec> cr(ctr[1]);
+    // ---- exact global order on the device; only the cell totals and kept candidates return
+    unsigned long long *nkept = S->ctr.as<unsigned long long>() + 2;
+    WZ_HIP(launch_stream_finalize(a, S->noff.as<uint64_t>(), S->loff.as<uint64_t>(), S->cell_tot.as<uint64_t>(),
+                                  ctr[1], S->kept.as<KeptCand>(), nkept, S->stream));
+    unsigned long long nk = 0;
+    std::vector<uint64_t> tot(3 * (size_t)ncells);
     std::vector<unsigned> rej(200);
-    WZ_HIP(hipMemcpy(inodes.data(), S->item_nodes.p, nitems * 8, hipMemcpyDeviceToHost));
-    WZ_HIP(hipMemcpy(ileaves.data(), S->item_leaves.p, nitems * 4, hipMemcpyDeviceToHost));
-    WZ_HIP(hipMemcpy(icands.data(), S->item_cands.p, nitems * 4, hipMemcpyDeviceToHost));
-    if (ctr[1]) WZ_HIP(hipMemcpy(cr.data(), S->cands.p, ctr[1] * sizeof(CandRec), hipMemcpyDeviceToHost));
-    WZ_HIP(hipMemcpy(rej.data(), S->rej.p, 200 * sizeof(unsigned), hipMemcpyDeviceToHost));
+    WZ_HIP(hipMemcpyAsync(&nk, nkept, 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(tot.data(), S->cell_tot.p, tot.size() * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(rej.data(), S->rej.p, 200 * sizeof(unsigned), hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipStreamSynchronize(S->stream));
+    std::vector<KeptCand> kc(nk);
+    if (nk) WZ_HIP(hipMemcpy(kc.data(), S->kept.p, nk * sizeof(KeptCand), hipMemcpyDeviceToHost));
     // refine the angle order for the next launch (decision-invariant)
     {
-      std::vector<unsigned long long> cnt(200, 0);
       for (int t = 0; t < 200; t++) S->angle_hist[S->angle_order[t]] += rej[t];
       std::vector<uint8_t> ord(S->angle_order);
       std::stable_sort(ord.begin(), ord.end(), [&](uint8_t x, uint8_t y) { return S->angle_hist[x] > S->angle_hist[y]; });
       S->angle_order = ord;
     }
-    std::sort(cr.begin(), cr.end(), [](const CandRec &x, const CandRec &y) {
-      if (x.item != y.item) return x.item < y.item;
-      return x.ell < y.ell;
+    std::sort(kc.begin(), kc.end(), [](const KeptCand &x, const KeptCand &y) {
+      if (x.cell != y.cell) return x.cell < y.cell;
+      return x.G < y.G;
     });
     out.cands.clear();
     out.cell_begin.assign(ncells + 1, 0);
@@ -353,32 +386,19 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     size_t pos = 0;
     for (int cs = 0; cs < ncells; cs++) {
       out.cell_begin[cs] = (long long)out.cands.size();
-      // item offsets in DFS order
-      std::vector<uint64_t> noff(npaths + 1, 0);
-      std::vector<uint64_t> loff(npaths + 1, 0);
-      for (int p = 0; p < npaths; p++) {
-        noff[p + 1] = noff[p] + inodes[(size_t)cs * npaths + p];
-        loff[p + 1] = loff[p] + ileaves[(size_t)cs * npaths + p];
-      }
-      const uint64_t total_nodes = noff[npaths];
-      const uint32_t lo = (uint32_t)cs * npaths, hi = lo + npaths;
-      while (pos < cr.size() && cr[pos].item < lo) pos++;
+      const uint64_t total_nodes = tot[3 * cs], total_leaves = tot[3 * cs + 1];
       long long taken = 0;
-      uint64_t last_g = 0;
-      uint32_t last_item = 0, last_leaf = 0;
-      bool cut_c = false, cut_n = false;
-      size_t q = pos;
-      for (; q < cr.size() && cr[q].item < hi; q++) {
-        const int p = (int)(cr[q].item - lo);
-        const uint64_t g = noff[p] + cr[q].ell;
-        if (o.node_budget > 0 && g >= (uint64_t)o.node_budget) { cut_n = true; break; }
-        if (o.cand_cap > 0 && taken >= o.cand_cap) { cut_c = true; break; }
-        out.cands.push_back(cr[q]);
-        taken++;
-        last_g = g; last_item = cr[q].item; last_leaf = cr[q].leaf;
+      uint64_t last_g = 0, last_leaf = 0;
+      while (pos < kc.size() && kc[pos].cell == (uint64_t)cs) {
+        if (!(o.cand_cap > 0 && taken >= o.cand_cap)) {
+          CandRec r;
+          r.X = kc[pos].X; r.Y = kc[pos].Y; r.ell = kc[pos].G; r.item = 0; r.leaf = 0;
+          out.cands.push_back(r);
+          taken++;
+          last_g = kc[pos].G; last_leaf = kc[pos].leaf;
+        }
+        pos++;
       }
-      while (q < cr.size() && cr[q].item < hi) q++;
-      pos = q;
       wzfh_cell_stats &stt = stats[cs];
       stt.cand = taken;
       stt.hall_ok = taken;
@@ -387,17 +407,16 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
         // the reference stops right after the cap-th candidate's leaf (rec() entry check)
         stt.capped = 1;
         stt.dfs_nodes = (long long)last_g + 1;
-        stt.leaves = (long long)(loff[last_item - lo] + last_leaf + 1);
-      } else if (o.node_budget > 0 && (cut_n || total_nodes > (uint64_t)o.node_budget)) {
-        stt.capped = 1;
+        stt.leaves = (long long)last_leaf + 1;
+      } else if (o.node_budget > 0 && total_nodes >= (uint64_t)o.node_budget) {
+        stt.capped = total_nodes > (uint64_t)o.node_budget ? 1 : 0;
         stt.dfs_nodes = o.node_budget;
-        stt.leaves = -1;
+        stt.leaves = stt.capped ? -1 : (long long)total_leaves;
       } else {
         stt.capped = 0;
         stt.dfs_nodes = (long long)total_nodes;
-        stt.leaves = (long long)loff[npaths];
+        stt.leaves = (long long)total_leaves;
       }
-      (void)cut_c;
       units += stt.dfs_nodes;
     }
     out.cell_begin[ncells] = (long long)out.cands.size();
@@ -634,7 +653,12 @@ int wzfh_npaf_check(int n, const int8_t *A, const int8_t *B, const int8_t *C, co
 }
 
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units) {
-  if (!p || !p->dev || kind < 0 || kind > 1) return WZFH_EINVAL;
+  if (!p || !p->dev || kind < 0 || kind > 5) return WZFH_EINVAL;
+  if (kind >= 2) {   // stream-kernel counters of the last launch (WZFH_TELEMETRY=1)
+    if (ms) *ms = (double)p->dev->last_telem[kind - 2];
+    if (units) *units = (long long)p->dev->last_telem[kind - 2];
+    return WZFH_OK;
+  }
   if (ms) *ms = p->dev->last_ms[kind];
   if (units) *units = p->dev->last_units[kind];
   return WZFH_OK;

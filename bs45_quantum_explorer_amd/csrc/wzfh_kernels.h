@@ -14,6 +14,14 @@ struct CandRec {
   uint32_t leaf;
 };
 
+// A candidate placed in its cell's global DFS order (after stream_finalize).
+struct KeptCand {
+  uint64_t X, Y;
+  uint64_t G;       // global DFS node index of the leaf inside its cell
+  uint64_t leaf;    // global leaf index inside its cell
+  uint64_t cell;    // cell slot of the launch
+};
+
 // C,D stream kernel arguments (count_pairs22 on the C,D side, wz_match.cpp:678-784).
 struct StreamArgs {
   int L, half, m, d0, npaths, window, ncells, nch0;
@@ -37,10 +45,16 @@ struct StreamArgs {
   uint64_t *item_nodes;        // [ncells * npaths]
   uint32_t *item_leaves;       // [ncells * npaths]
   uint32_t *item_cands;        // [ncells * npaths]
+  uint32_t *item_done;         // [ncells * npaths] set once the item's counts are final
+  int *cell_lock;              // [ncells] completed-prefix tracker (try-lock)
+  int *cell_pp;                // [ncells] items in the completed prefix
+  unsigned long long *cell_pn, *cell_pc;   // [ncells] its node and candidate sums
   CandRec *cands;
   unsigned long long *ncands;
   unsigned long long cand_capacity;
   unsigned int *angle_rej;     // [200] first-rejecting-angle histogram (angle order tuning)
+  unsigned long long *telem;   // optional per-launch counters (null = off): [0] cycles in the
+                               // leaf test, [1] total wave cycles, [2] flushes, [3] leaves tested
 };
 
 // A,B completer kernel arguments (fh_complete_ab, wz_match.cpp:1092-1341).

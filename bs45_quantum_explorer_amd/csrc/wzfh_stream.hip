@@ -131,6 +131,41 @@ __device__ bool hall_pass(uint64_t X, uint64_t Y, bool active, int L, double lim
   return pass;
 }
 
+// Completed-prefix tracker of one cell. Items of a cell finish in any order; the lane
+// that takes the cell's lock walks the run of finished items after the current prefix,
+// summing their nodes and candidates, and publishes the cell's cut as soon as the sums
+// reach the node budget or candidate cap: every later item is then certainly beyond the
+// cut and is skipped or abandoned. The lock is only ever tried (never waited on), and a
+// lane that sees new finished items after releasing it tries again a bounded number of
+// times. The exact offsets are recomputed by stream_finalize_kernel, so this walk only
+// decides how early a cell stops.
+__device__ void advance_prefix(const StreamArgs &a, int cs) {
+  const size_t base = (size_t)cs * a.npaths;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    if (atomicCAS(&a.cell_lock[cs], 0, 1) != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    int pp = a.cell_pp[cs];
+    unsigned long long pn = a.cell_pn[cs], pc = a.cell_pc[cs];
+    bool crossed = false;
+    while (pp < a.npaths) {
+      if (!__hip_atomic_load(&a.item_done[base + pp], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) break;
+      pn += a.item_nodes[base + pp];
+      pc += __hip_atomic_load(&a.item_cands[base + pp], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      pp++;
+      if ((a.node_budget > 0 && pn >= (unsigned long long)a.node_budget) ||
+          (a.cand_cap > 0 && pc >= (unsigned long long)a.cand_cap)) { crossed = true; break; }
+    }
+    a.cell_pp[cs] = crossed ? a.npaths : pp;   // a crossed cell's walk is over
+    a.cell_pn[cs] = pn;
+    a.cell_pc[cs] = pc;
+    if (crossed) atomicMin(&a.cut[cs], pp - 1);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    atomicExch(&a.cell_lock[cs], 0);
+    if (crossed || pp >= a.npaths) return;
+    if (!__hip_atomic_load(&a.item_done[base + pp], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) return;
+  }
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
@@ -164,6 +199,8 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
   bool more = true;
   int qn = 0;
   unsigned long long moves = 0;
+  const bool telem = a.telem != nullptr;
+  unsigned long long t_start = telem ? clock64() : 0, t_hall = 0, n_flush = 0, n_tested = 0;
 
   while (true) {
     // ---- 1. idle lanes take the next items (one atomic per wave)
@@ -283,12 +320,6 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
       if (stop) { atomicMin(&a.cut[cs], p); st = ST_DONE; }
       else if ((++moves & 255) == 0 && __hip_atomic_load(&a.cut[cs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < p) st = ST_DONE;
     }
-    if (st == ST_DONE) {
-      a.item_nodes[item] = ell;
-      a.item_leaves[item] = nleaf;
-      st = ST_IDLE;
-    }
-
     // ---- 3. queue the new leaves
     const unsigned long long lm = __ballot(has_leaf);
     if (lm) {
@@ -296,13 +327,17 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
       if (has_leaf) q[slot] = lf;
       qn += __popcll(lm);
     }
-    const bool idle_all = !more && (__ballot(st != ST_IDLE) == 0);
-    // ---- 4. flush 64 leaves (or the tail) through the FP64 hall test
-    if (qn >= 64 || (idle_all && qn > 0)) {
+    // ---- 4. flush 64 leaves at a time through the FP64 hall test. A finished item's
+    // candidate count must be final before the item is published, so a wave holding such
+    // an item (or out of work) drains its whole queue.
+    const bool drain = (__ballot(st == ST_DONE && nleaf > 0) != 0) ||
+                       (!more && __ballot(st == ST_EXPAND || st == ST_BACK) == 0);
+    while (qn >= 64 || (drain && qn > 0)) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       const int nf = qn < 64 ? qn : 64;
       const bool have = lane < nf;
+      const unsigned long long t0 = telem ? clock64() : 0;
       Leaf e;
       if (have) e = q[lane];
       else { e.X = e.Y = e.ell = 0; e.item = e.leaf = 0; }
@@ -323,6 +358,7 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
           }
         }
       }
+      if (telem) { t_hall += clock64() - t0; n_flush++; n_tested += nf; }
       const int rem = qn - nf;
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -331,7 +367,21 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
       __builtin_amdgcn_wave_barrier();
       qn = rem;
     }
-    if (idle_all && qn == 0) break;
+    // ---- 5. publish finished items and advance their cells' completed prefix
+    if (st == ST_DONE) {
+      a.item_nodes[item] = ell;
+      a.item_leaves[item] = nleaf;
+      __hip_atomic_store(&a.item_done[item], (uint32_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.node_budget > 0 || a.cand_cap > 0) advance_prefix(a, cs);
+      st = ST_IDLE;
+    }
+    if (!more && qn == 0 && __ballot(st != ST_IDLE) == 0) break;
+  }
+  if (telem && lane == 0) {
+    atomicAdd(&a.telem[0], t_hall);
+    atomicAdd(&a.telem[1], (unsigned long long)(clock64() - t_start));
+    atomicAdd(&a.telem[2], n_flush);
+    atomicAdd(&a.telem[3], n_tested);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 200; i += blockDim.x)
@@ -341,8 +391,66 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
 }  // namespace wzfh
 
 namespace wzfh {
+
+// Exact global order after the stream: one block per cell scans the cell's items in path
+// order (the reference's DFS order) into node / leaf offsets and cell totals; then every
+// candidate gets its global DFS node index G and leaf index, and the ones that can be in
+// the cell's stream (path not beyond the cut, G below the node budget) are compacted.
+__global__ __launch_bounds__(256) void stream_scan_kernel(StreamArgs a, uint64_t *noff, uint64_t *loff,
+                                                          uint64_t *cell_tot) {
+  __shared__ uint64_t s_n[256], s_l[256], s_c[256];
+  const int cs = blockIdx.x, t = threadIdx.x;
+  const size_t base = (size_t)cs * a.npaths;
+  uint64_t carry_n = 0, carry_l = 0, carry_c = 0;
+  for (int p0 = 0; p0 < a.npaths; p0 += 256) {
+    const int p = p0 + t;
+    const uint64_t vn = p < a.npaths ? a.item_nodes[base + p] : 0;
+    const uint64_t vl = p < a.npaths ? a.item_leaves[base + p] : 0;
+    const uint64_t vc = p < a.npaths ? a.item_cands[base + p] : 0;
+    s_n[t] = vn; s_l[t] = vl; s_c[t] = vc;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {   // inclusive Hillis-Steele scan
+      uint64_t xn = 0, xl = 0, xc = 0;
+      if (t >= off) { xn = s_n[t - off]; xl = s_l[t - off]; xc = s_c[t - off]; }
+      __syncthreads();
+      s_n[t] += xn; s_l[t] += xl; s_c[t] += xc;
+      __syncthreads();
+    }
+    if (p < a.npaths) { noff[base + p] = carry_n + s_n[t] - vn; loff[base + p] = carry_l + s_l[t] - vl; }
+    carry_n += s_n[255]; carry_l += s_l[255]; carry_c += s_c[255];
+    __syncthreads();
+  }
+  if (t == 0) { cell_tot[3 * cs] = carry_n; cell_tot[3 * cs + 1] = carry_l; cell_tot[3 * cs + 2] = carry_c; }
+}
+
+__global__ __launch_bounds__(256) void stream_keep_kernel(StreamArgs a, const uint64_t *noff, const uint64_t *loff,
+                                                          unsigned long long ncands, KeptCand *kept,
+                                                          unsigned long long *nkept) {
+  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncands) return;
+  const CandRec r = a.cands[i];
+  const int cs = (int)(r.item / (uint32_t)a.npaths), p = (int)(r.item % (uint32_t)a.npaths);
+  if (p > a.cut[cs]) return;
+  const uint64_t G = noff[r.item] + r.ell;
+  if (a.node_budget > 0 && G >= (uint64_t)a.node_budget) return;
+  const unsigned long long k = atomicAdd(nkept, 1ull);
+  KeptCand o;
+  o.X = r.X; o.Y = r.Y; o.G = G; o.leaf = loff[r.item] + r.leaf; o.cell = (uint64_t)cs;
+  kept[k] = o;
+}
+
 hipError_t launch_stream(const StreamArgs &a, int blocks, hipStream_t st) {
   hipLaunchKernelGGL(stream_kernel, dim3(blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_stream_finalize(const StreamArgs &a, uint64_t *noff, uint64_t *loff, uint64_t *cell_tot,
+                                  unsigned long long ncands, KeptCand *kept, unsigned long long *nkept,
+                                  hipStream_t st) {
+  hipLaunchKernelGGL(stream_scan_kernel, dim3(a.ncells), dim3(256), 0, st, a, noff, loff, cell_tot);
+  if (ncands) {
+    const unsigned blocks = (unsigned)((ncands + 255) / 256);
+    hipLaunchKernelGGL(stream_keep_kernel, dim3(blocks), dim3(256), 0, st, a, noff, loff, ncands, kept, nkept);
+  }
   return hipGetLastError();
 }
 }  // namespace wzfh

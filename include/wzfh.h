@@ -176,7 +176,9 @@ long long wzfh_flat_score(int n, uint64_t C, uint64_t D);
 /* ---- Device timing (for measurement) ----------------------------------------- */
 /* Average device duration of the named kernel over the last wzfh_search_cells /
  * wzfh_stream_cells / wzfh_complete call, measured with HIP events on the stream the
- * kernel runs on. kind: 0 = stream kernel, 1 = completer kernel. */
+ * kernel runs on. kind: 0 = stream kernel, 1 = completer kernel (units = DFS nodes).
+ * kind 2..5 (with WZFH_TELEMETRY=1 in the environment): the last stream launch's wave
+ * cycles in the leaf test, total wave cycles, leaf-test flushes, leaves tested. */
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units);
 
 #ifdef __cplusplus

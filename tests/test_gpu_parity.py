@@ -121,3 +121,67 @@ for c in CASES:
 @pytest.mark.parametrize('key', sorted(GROUPS), ids=lambda k: f'n{k[0]}-{k[1]}-k{k[2]}')
 def test_complete_only_matches_reference(key):
     _complete_group(GROUPS[key])
+
+
+@pytest.mark.parametrize('n,sig,budgets', [(13, (2, 4, 3, 5), (0, 30)), (19, (0, 2, 5, 7), (0, 500)),
+                                           (12, (1, 7, 0, 0), (0, 7))])
+def test_in_cell_completion_with_profile_rows_matches_oracle(n, sig, budgets):
+    """The in-stream completer (cell's A,B profile rows applied, wz_match.cpp:2505-2530) against
+    the oracle, which is pinned to the reference's full arm runs (firsthit_runs.json)."""
+    from oracle_lib import OracleProblem
+    P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1)
+    O = OracleProblem(n, sig, canon=1, order=1)
+    try:
+        cells = P.live_cells()[:60]
+        st, offs, cd = P.stream(cells, cand_cap=12)
+        cell_of = [cells[i] for i in range(len(cells)) for _ in range(offs[i], offs[i + 1])]
+        for budget in budgets:
+            comp = P.complete(cd, budget=budget, cell_of=cell_of)
+            for j, (c, pi) in enumerate(zip(comp, cell_of)):
+                Cs, Ds = wz.unpack(cd[j][0], n), wz.unpack(cd[j][1], n)
+                r, nodes, A, B = O.complete(Cs, Ds, budget, pi=pi)
+                assert (c.verdict, c.nodes) == (r, nodes), (n, sig, budget, pi, j)
+                if r == 0:
+                    assert list(wz.unpack(c.A, n + 1)) == list(A) and list(wz.unpack(c.B, n + 1)) == list(B)
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize('split,window', [(2, 1), (3, 4), (4, 64)])
+def test_stream_node_budget_matches_oracle(split, window):
+    """Serving policy: a cell's stream is cut at a DFS node budget (nodes with DFS index below
+    the budget are visited). The oracle restates the cut (wzo_stream_cell_budget)."""
+    from oracle_lib import OracleProblem
+    n, sig = 19, (2, 4, 3, 7)
+    P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1)
+    O = OracleProblem(n, sig, canon=1, order=1)
+    try:
+        cells = P.live_cells()[:48]
+        for budget in (1, 37, 500, 4096, 100000):
+            st, offs, cd = P.stream(cells, node_budget=budget, split_depth=split, window=window)
+            for i, pi in enumerate(cells):
+                got, cands, ost = O.stream_budget(pi, 0, budget, 4096)
+                assert st[i].cand == got and st[i].dfs_nodes == ost.dfs_nodes, (budget, pi)
+                if ost.dfs_nodes < budget:
+                    assert st[i].leaves == ost.leaves
+                assert wz.dump_lines(n, cd[offs[i]:offs[i + 1]]) == [
+                    ''.join(f'{int(v)} ' for v in c[0]) + '| ' + ''.join(f'{int(v)} ' for v in c[1]) + '\n'
+                    for c in cands], (budget, pi)
+    finally:
+        P.close()
+
+
+def test_stream_cap_and_budget_together_match_oracle():
+    from oracle_lib import OracleProblem
+    n, sig = 13, (2, 4, 3, 5)
+    P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=0, order=1)
+    O = OracleProblem(n, sig, canon=0, order=1)
+    try:
+        cells = P.live_cells()[:80]
+        for cap, budget in ((3, 200), (1, 50), (10, 10 ** 6)):
+            st, offs, cd = P.stream(cells, cand_cap=cap, node_budget=budget, split_depth=2, window=2)
+            for i, pi in enumerate(cells):
+                got, cands, ost = O.stream_budget(pi, cap, budget, 64)
+                assert (st[i].cand, st[i].dfs_nodes) == (got, ost.dfs_nodes), (cap, budget, pi)
+    finally:
+        P.close()
