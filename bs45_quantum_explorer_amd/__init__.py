@@ -19,7 +19,8 @@ HIT, PREREJECT, CLEAN_NO, ABORTED, INTERNAL = 0, 1, 2, 3, 4
 
 EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy', 'wzfh_last_error',
            'wzfh_num_cells', 'wzfh_cell_info', 'wzfh_live_cells', 'wzfh_order_digest', 'wzfh_stream_cells', 'wzfh_complete',
-           'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms')
+           'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms',
+           'wzfh_serve_open', 'wzfh_serve_submit', 'wzfh_serve_poll', 'wzfh_serve_close')
 
 
 class WzfhError(RuntimeError):
@@ -59,6 +60,16 @@ class SearchResult(C.Structure):
                 ('A', C.c_uint64), ('B', C.c_uint64), ('C', C.c_uint64), ('D', C.c_uint64)]
 
 
+class ServeOpts(C.Structure):
+    _fields_ = [('search', SearchOpts), ('max_batch', C.c_int), ('reserved', C.c_int)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [('ticket', C.c_longlong), ('problem', C.c_int), ('cell', C.c_int), ('status', C.c_int),
+                ('batch_size', C.c_int), ('submit_s', C.c_double), ('start_s', C.c_double),
+                ('finish_s', C.c_double), ('result', SearchResult)]
+
+
 _lib = None
 
 
@@ -87,6 +98,10 @@ def lib():
         L.wzfh_flat_score.argtypes = [C.c_int, C.c_uint64, C.c_uint64]
         L.wzfh_flat_score.restype = C.c_longlong
         L.wzfh_last_kernel_ms.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        L.wzfh_serve_open.argtypes = [vp, C.c_int, C.POINTER(ServeOpts), C.POINTER(vp)]
+        L.wzfh_serve_submit.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong)]
+        L.wzfh_serve_poll.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
+        L.wzfh_serve_close.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -205,6 +220,38 @@ class Problem:
         ms, units = C.c_double(), C.c_longlong()
         _check(lib().wzfh_last_kernel_ms(self.h, kind, C.byref(ms), C.byref(units)))
         return ms.value, units.value
+
+
+class Server:
+    """Continuous-batching cell-search server over a list of Problems (wzfh_serve_*)."""
+
+    def __init__(self, problems, top_k, budget, cand_cap=0, node_budget=0, max_batch=256, split_depth=0,
+                 window=0):
+        self.problems = list(problems)
+        arr = (C.c_void_p * len(self.problems))(*[p.h for p in self.problems])
+        opts = ServeOpts(SearchOpts(StreamOpts(cand_cap, node_budget, split_depth, window), top_k, 1, budget),
+                         max_batch, 0)
+        h = C.c_void_p()
+        _check(lib().wzfh_serve_open(arr, len(self.problems), C.byref(opts), C.byref(h)))
+        self.h = h
+
+    def submit(self, problem, cell):
+        t = C.c_longlong()
+        _check(lib().wzfh_serve_submit(self.h, problem, cell, C.byref(t)))
+        return t.value
+
+    def poll(self, max_results=1024, min_results=1, timeout_ms=1000):
+        out = (ServeResult * max_results)()
+        k = lib().wzfh_serve_poll(self.h, out, max_results, min_results, timeout_ms)
+        if k < 0:
+            _check(k)
+        return [out[i] for i in range(k)]
+
+    def close(self):
+        if getattr(self, 'h', None):
+            rc = lib().wzfh_serve_close(self.h)
+            self.h = None
+            _check(rc)
 
 
 def flat_score(n, Cw, Dw):

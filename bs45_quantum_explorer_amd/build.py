@@ -20,7 +20,7 @@ ARCH = os.environ.get('WZFH_ARCH', 'gfx950')
 
 HIP_FLAGS = ['-O3', '-std=c++17', f'--offload-arch={ARCH}', '-fPIC', '-ffp-contract=off',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wno-unused-result']
-SOURCES = ['wzfh_stream.hip', 'wzfh_complete.hip', 'wzfh_api.hip', 'wzfh_setup.cpp']
+SOURCES = ['wzfh_stream.hip', 'wzfh_complete.hip', 'wzfh_api.hip', 'wzfh_setup.cpp', 'wzfh_serve.cpp']
 
 
 def hipcc():
@@ -34,7 +34,7 @@ def _compile(src):
     out = OBJ / (Path(src).stem + '.o')
     cmd = [hipcc(), *HIP_FLAGS, '-c', str(CSRC / src), '-o', str(out)]
     if src.endswith('.cpp'):
-        cmd = [hipcc(), '-O3', '-std=c++17', '-fPIC', '-ffp-contract=off', '-c', str(CSRC / src), '-o', str(out)]
+        cmd = [hipcc(), '-O3', '-std=c++17', '-fPIC', '-pthread', '-ffp-contract=off', '-c', str(CSRC / src), '-o', str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f'compile failed: {" ".join(cmd)}\n{r.stderr}')
@@ -46,7 +46,7 @@ def build(verbose=True):
     with ThreadPoolExecutor(4) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix('.so.tmp')
-    cmd = [hipcc(), f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', str(tmp), *map(str, objs)]
+    cmd = [hipcc(), f'--offload-arch={ARCH}', '-shared', '-fPIC', '-pthread', '-o', str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f'link failed: {" ".join(cmd)}\n{r.stderr}')

@@ -267,6 +267,7 @@ static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncel
 static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o,
                       wzfh_cell_stats *stats, StreamOut &out) {
   DeviceState *S = P.dev;
+  WZ_HIP(hipSetDevice(S->device));   // callers may be any host thread (serving workers)
   StreamArgs a = S->base;
   const int d0 = auto_split(P, o, ncells);
   const int npaths = a.nch0 * (1 << (3 * (d0 - 1)));
@@ -432,6 +433,7 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
                         wzfh_completion *out) {
   DeviceState *S = P.dev;
   if (ncand <= 0) return WZFH_OK;
+  WZ_HIP(hipSetDevice(S->device));
   std::vector<int> roff(ncand, -1), rlen(ncand, 0);
   int maxrows = 0;
   for (int i = 0; i < ncand; i++) {

@@ -165,6 +165,34 @@ typedef struct {
 int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts,
                       wzfh_search_result *out);
 
+/* ---- Serving: continuous batching of cell-search requests ------------------------
+ * No reference counterpart (the reference runs fixed arm processes). Requests name a
+ * problem (an initialized (n, class) handle, kept device-resident) and a cell; one worker
+ * per problem admits whatever has arrived, up to max_batch cells, into one batched search
+ * on the problem's own stream, so batches of different problems overlap on the device.
+ * A result equals the same cell searched alone. Handles passed to a server must not be
+ * used by the caller until wzfh_serve_close returns. */
+typedef struct wzfh_server wzfh_server;
+typedef struct {
+  wzfh_search_opts search;   /* per-cell unit (batched flag ignored: always batched) */
+  int max_batch;             /* cells admitted into one launch per problem            */
+  int reserved;
+} wzfh_serve_opts;
+typedef struct {
+  long long ticket;
+  int problem, cell;
+  int status;                /* WZFH_OK or the error of the batch                      */
+  int batch_size;            /* cells in the launch that served this request          */
+  double submit_s, start_s, finish_s;   /* seconds since the server opened             */
+  wzfh_search_result result;
+} wzfh_serve_result;
+int wzfh_serve_open(wzfh_problem **problems, int nproblems, const wzfh_serve_opts *opts, wzfh_server **out);
+int wzfh_serve_submit(wzfh_server *s, int problem, int cell, long long *ticket);
+/* Waits up to timeout_ms for at least min_results finished requests; returns how many
+ * were written to out (<= max), or a negative error. */
+int wzfh_serve_poll(wzfh_server *s, wzfh_serve_result *out, int max, int min_results, int timeout_ms);
+int wzfh_serve_close(wzfh_server *s);
+
 /* ---- Exact checks ----------------------------------------------------------- */
 /* Summed NPAF over s = 1..n of int8 +1/-1 sequences (npaf_at, wz_match.cpp:184-192,
  * and the VERIFY line :2640-2646). *max_abs gets max_s |NPAF[s]|. */
