@@ -61,7 +61,7 @@ struct DeviceState {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // problem constants
-  DBuf hcos, hsin, order, rej, outer2, outer3, posidx, rows, kab;
+  DBuf hcos, hsin, order, rej, outer2, outer3, posidx, rows, kab, hall_frag;
   StreamArgs base{};                       // tables + constants of the stream kernel
   std::vector<uint8_t> angle_order;        // current evaluation order (angle j-1)
   std::vector<unsigned long long> angle_hist;
@@ -73,7 +73,8 @@ struct DeviceState {
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
-  unsigned long long last_telem[4] = {0, 0, 0, 0};
+  unsigned long long last_telem[6] = {0, 0, 0, 0, 0, 0};
+  double hall_eps = 0.0;           // margin of the matrix-core leaf test
   // the last stream result, reused when the caller asks again for the same cells and
   // options (count first, then fetch) so the kernel runs once
   std::vector<int> memo_cells;
@@ -164,6 +165,56 @@ static void build_stream_tables(const wzfh_problem &P, StreamArgs &a) {
     a.tab0_eq[idx] = (uint16_t)m16;
   }
   a.lim = 4.0 * P.n + 2.0 + 0.5;
+  a.hall_nkb = (a.L + 31) / 32;
+}
+
+// MFMA A-operand fragments of the leaf-test angle table (hall_mfma in wzfh_stream.hip):
+// entry (((g*2 + t)*2 + kb)*2 + part)*64 + lane holds 8 fp16 values of row a = 16g + lane%16
+// (angle j = a+1 for a < 100, j = 200 for a = 100, zero rows above), positions
+// k = 32kb + 8*(lane/16) + e, of cos (t = 0) or sin (t = 1): part 0 = fp16(v), part 1 =
+// fp16(v - part 0). Returns max |v - hi - lo| over the table (exact in FP64).
+static double build_hall_frag(const wzfh_problem &P, std::vector<_Float16> &out) {
+  constexpr int kGroups = 7;
+  out.assign((size_t)kGroups * 2 * 2 * 2 * 64 * 8, (_Float16)0.0f);
+  double worst = 0.0;
+  for (int g = 0; g < kGroups; g++)
+    for (int t = 0; t < 2; t++)
+      for (int kb = 0; kb < 2; kb++)
+        for (int lane = 0; lane < 64; lane++)
+          for (int e = 0; e < 8; e++) {
+            const int row = 16 * g + (lane & 15), k = 32 * kb + 8 * (lane >> 4) + e;
+            double v = 0.0;
+            if (row <= 100) {
+              const int j = row < 100 ? row + 1 : 200;
+              v = t == 0 ? P.hcos[j][k] : P.hsin[j][k];
+            }
+            const _Float16 hi = (_Float16)v;
+            const _Float16 lo = (_Float16)(v - (double)hi);
+            worst = std::max(worst, std::fabs((v - (double)hi) - (double)lo));
+            out[((((size_t)(g * 2 + t) * 2 + kb) * 2 + 0) * 64 + lane) * 8 + e] = hi;
+            out[((((size_t)(g * 2 + t) * 2 + kb) * 2 + 1) * 64 + lane) * 8 + e] = lo;
+          }
+  return worst;
+}
+
+// Margin of the matrix-core leaf test. A lane's sum r = sum_k s_k (hi_k + lo_k) differs from
+// the exact sum over the FP64 table by at most delta = L*split + acc, where acc bounds the
+// FP32 accumulation of 2L exact products (every partial sum is at most L(1 + 2^-11) in
+// magnitude; 2^-23 per addition also covers truncating adders). The reference's own FP64
+// sums are within 2L^2 * 2^-53 of exact. For the four sums of one (leaf, angle),
+// |S' - S| <= 2*delta*sum|r_t| + 4*delta^2 <= 4*delta*sqrt(S) + 4*delta^2 (Cauchy-Schwarz), and
+// forming S in FP32 adds at most 8 * S * 2^-23; only S up to lim + 1 matters for either
+// certain decision. The margin used is twice that bound.
+static void hall_thresholds(int L, double lim, double split_err, float &hi, float &lo, double &eps) {
+  const double u23 = std::ldexp(1.0, -23);
+  const double acc = 2.0 * L * (L * (1.0 + std::ldexp(1.0, -11))) * u23;
+  const double ref = 2.0 * L * L * std::ldexp(1.0, -53);
+  const double delta = L * split_err + acc + ref;
+  const double smax = lim + 1.0;
+  const double bound = 4.0 * delta * std::sqrt(smax) + 4.0 * delta * delta + 8.0 * smax * u23;
+  eps = 2.0 * bound;
+  hi = std::nextafter((float)(lim + eps), INFINITY);
+  lo = std::nextafter((float)(lim - eps), -INFINITY);
 }
 
 // Kab(D, s): 2 * number of shift-s pairs of an A,B of length L not yet fully placed
@@ -226,13 +277,20 @@ int device_init(wzfh_problem &P) {
   if (S->kab.ensure(S->kab_host.size())) return WZFH_ENOMEM;
   WZ_HIP(hipMemcpy(S->kab.p, S->kab_host.data(), S->kab_host.size(), hipMemcpyHostToDevice));
   build_stream_tables(P, S->base);
+  {
+    std::vector<_Float16> frag;
+    const double split_err = build_hall_frag(P, frag);
+    if (S->hall_frag.ensure(frag.size() * sizeof(_Float16))) return WZFH_ENOMEM;
+    WZ_HIP(hipMemcpy(S->hall_frag.p, frag.data(), frag.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+    hall_thresholds(S->base.L, S->base.lim, split_err, S->base.hall_hi, S->base.hall_lo, S->hall_eps);
+  }
   return WZFH_OK;
 }
 
 void device_free(wzfh_problem &P) {
   DeviceState *S = P.dev;
   if (!S) return;
-  for (DBuf *b : {&S->telem, &S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->posidx, &S->rows, &S->kab,
+  for (DBuf *b : {&S->telem, &S->hall_frag, &S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->posidx, &S->rows, &S->kab,
                   &S->cell_res, &S->cell_rootok, &S->cut, &S->item_nodes, &S->item_leaves, &S->item_cands,
                   &S->cands, &S->ctr, &S->item_done, &S->cell_lock, &S->cell_pp, &S->cell_pn, &S->cell_pc,
                   &S->noff, &S->loff, &S->cell_tot, &S->kept, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
@@ -322,6 +380,8 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     a.cell_res = S->cell_res.as<uint64_t>();
     a.cell_rootok = S->cell_rootok.as<int>();
     a.hcos = S->hcos.as<double>(); a.hsin = S->hsin.as<double>();
+    a.hall_frag = S->hall_frag.p;
+    a.hall_verify = (getenv("WZFH_HALL_VERIFY") && atoi(getenv("WZFH_HALL_VERIFY"))) ? 1 : 0;
     a.angle_order = S->order.as<uint8_t>();
     a.work_ctr = S->ctr.as<unsigned long long>();
     a.ncands = S->ctr.as<unsigned long long>() + 1;
@@ -355,7 +415,7 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     float ms = 0;
     WZ_HIP(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
     S->last_ms[0] = ms;
-    if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 32, hipMemcpyDeviceToHost));
+    if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 48, hipMemcpyDeviceToHost));
     if (ctr[1] > cap) { cap = ctr[1] + (ctr[1] >> 2) + 1024; continue; }
     // ---- exact global order on the device; only the cell totals and kept candidates return
     unsigned long long *nkept = S->ctr.as<unsigned long long>() + 2;
@@ -655,7 +715,12 @@ int wzfh_npaf_check(int n, const int8_t *A, const int8_t *B, const int8_t *C, co
 }
 
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units) {
-  if (!p || !p->dev || kind < 0 || kind > 5) return WZFH_EINVAL;
+  if (!p || !p->dev || kind < 0 || kind > 8) return WZFH_EINVAL;
+  if (kind == 8) {   // margin of the matrix-core leaf test
+    if (ms) *ms = p->dev->hall_eps;
+    if (units) *units = 0;
+    return WZFH_OK;
+  }
   if (kind >= 2) {   // stream-kernel counters of the last launch (WZFH_TELEMETRY=1)
     if (ms) *ms = (double)p->dev->last_telem[kind - 2];
     if (units) *units = (long long)p->dev->last_telem[kind - 2];

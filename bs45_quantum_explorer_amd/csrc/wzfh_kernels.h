@@ -53,8 +53,17 @@ struct StreamArgs {
   unsigned long long *ncands;
   unsigned long long cand_capacity;
   unsigned int *angle_rej;     // [200] first-rejecting-angle histogram (angle order tuning)
+  // Leaf test on the matrix cores: A-operand fragments of the split angle table (see
+  // hall_mfma in wzfh_stream.hip), the certain-reject / uncertain thresholds on the pair
+  // energy, the number of 16-position K blocks covering L, and a verify switch that also
+  // runs the FP64 test on every leaf and counts disagreements with the certain decisions.
+  const void *hall_frag;
+  float hall_hi, hall_lo;
+  int hall_nkb;
+  int hall_verify;
   unsigned long long *telem;   // optional per-launch counters (null = off): [0] cycles in the
-                               // leaf test, [1] total wave cycles, [2] flushes, [3] leaves tested
+                               // leaf test, [1] total wave cycles, [2] flushes, [3] leaves tested,
+                               // [4] leaves rechecked in FP64, [5] disagreements (verify mode)
 };
 
 // A,B completer kernel arguments (fh_complete_ab, wz_match.cpp:1092-1341).

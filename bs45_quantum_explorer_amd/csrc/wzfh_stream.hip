@@ -131,6 +131,92 @@ __device__ bool hall_pass(uint64_t X, uint64_t Y, bool active, int L, double lim
   return pass;
 }
 
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+// 8 consecutive positions base..base+7 of a packed sequence as fp16 +-1 (bit set = -1),
+// 0 at positions >= L.
+__device__ __forceinline__ half8_t seq_frag(uint64_t bits, int base, int L) {
+  const unsigned byte = (unsigned)(bits >> base) & 0xFFu;
+  const int nv = L - base;
+  const unsigned vm = nv >= 8 ? 0xFFu : (nv <= 0 ? 0u : ((1u << nv) - 1u));
+  u32x4_t w;
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const unsigned e0 = 2 * h, e1 = 2 * h + 1;
+    const unsigned h0 = ((vm >> e0) & 1u) ? (0x3C00u | (((byte >> e0) & 1u) << 15)) : 0u;
+    const unsigned h1 = ((vm >> e1) & 1u) ? (0x3C00u | (((byte >> e1) & 1u) << 15)) : 0u;
+    w[h] = h0 | (h1 << 16);
+  }
+  return __builtin_bit_cast(half8_t, w);
+}
+
+constexpr int kHallGroups = 7;   // 16-angle row groups covering the 101 distinct angles
+
+// Leaf test of the 64 queued leaves on the matrix cores. The reference rejects a leaf iff
+// some angle j = 1..200 has P_X + P_Y > lim in FP64 (P_X > lim alone implies it), and
+// angles j and 200-j carry the same energy, so 101 angles (j = 1..100 and 200) decide.
+// D[angle][sequence] = T[angle][pos] * S[pos][sequence] runs as v_mfma_f32_16x16x32_f16
+// with T = hi + lo, a two-term fp16 split of the FP64 table (|T - hi - lo| <= 2^-24), and
+// the sequences as exact fp16 +-1. Rows are angles (7 groups of 16, cos and sin tiles),
+// columns are the X and Y sequences of 16 leaves, so one lane holds the four sums of one
+// (leaf, angle) and forms S = xc^2 + yc^2 + xs^2 + ys^2 in FP32. The host bounds
+// |S - S_fp64| by a margin eps (table split, FP32 accumulation and evaluation, and the
+// reference's own FP64 rounding) and passes hall_hi = lim + eps, hall_lo = lim - eps:
+// S > hall_hi at any angle is a certain reject, S <= hall_lo at every angle a certain
+// pass, anything else is uncertain and is decided by the FP64 test. Returns, for leaf
+// `lane`: bit0 = certain reject, bit1 = some angle above hall_lo. Must be called with the
+// whole wave active.
+__device__ unsigned hall_mfma(const Leaf *q, int L, int nkb, const u32x4_t *frag, float thr_hi, float thr_lo,
+                              int lane) {
+  unsigned mine = 0;
+#pragma unroll 1
+  for (int lg = 0; lg < 4; lg++) {
+    const Leaf &f = q[lg * 16 + (lane & 15)];
+    const uint64_t xb = f.X, yb = f.Y;
+    const int base = 8 * (lane >> 4);
+    const half8_t bx0 = seq_frag(xb, base, L), by0 = seq_frag(yb, base, L);
+    const half8_t bx1 = seq_frag(xb, 32 + base, L), by1 = seq_frag(yb, 32 + base, L);
+    unsigned flags = 0;
+#pragma unroll 1
+    for (int g = 0; g < kHallGroups; g++) {
+      floatx4_t xc = {}, yc = {}, xs = {}, ys = {};
+      const u32x4_t *fg = frag + (size_t)g * 2 * 2 * 2 * 64 + lane;   // [t][kb][part][lane]
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        const half8_t ac = __builtin_bit_cast(half8_t, fg[(0 * 4 + 0 * 2 + p) * 64]);
+        const half8_t as = __builtin_bit_cast(half8_t, fg[(1 * 4 + 0 * 2 + p) * 64]);
+        xc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ac, bx0, xc, 0, 0, 0);
+        yc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ac, by0, yc, 0, 0, 0);
+        xs = __builtin_amdgcn_mfma_f32_16x16x32_f16(as, bx0, xs, 0, 0, 0);
+        ys = __builtin_amdgcn_mfma_f32_16x16x32_f16(as, by0, ys, 0, 0, 0);
+      }
+      if (nkb > 1) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+          const half8_t ac = __builtin_bit_cast(half8_t, fg[(0 * 4 + 1 * 2 + p) * 64]);
+          const half8_t as = __builtin_bit_cast(half8_t, fg[(1 * 4 + 1 * 2 + p) * 64]);
+          xc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ac, bx1, xc, 0, 0, 0);
+          yc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ac, by1, yc, 0, 0, 0);
+          xs = __builtin_amdgcn_mfma_f32_16x16x32_f16(as, bx1, xs, 0, 0, 0);
+          ys = __builtin_amdgcn_mfma_f32_16x16x32_f16(as, by1, ys, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float S = ((xc[r] * xc[r] + yc[r] * yc[r]) + xs[r] * xs[r]) + ys[r] * ys[r];
+        flags |= (S > thr_hi ? 1u : 0u) | (S > thr_lo ? 2u : 0u);
+      }
+    }
+    // lanes l, l^16, l^32, l^48 hold the other angle rows of the same leaf
+    flags |= (unsigned)__shfl_xor((int)flags, 16);
+    flags |= (unsigned)__shfl_xor((int)flags, 32);
+    if ((lane >> 4) == lg) mine = flags;
+  }
+  return mine;
+}
+
 // Completed-prefix tracker of one cell. Items of a cell finish in any order; the lane
 // that takes the cell's lock walks the run of finished items after the current prefix,
 // summing their nodes and candidates, and publishes the cell's cut as soon as the sums
@@ -168,7 +254,7 @@ __device__ void advance_prefix(const StreamArgs &a, int cs) {
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void stream_kernel(StreamArgs a) {
   __shared__ uint8_t s_ne[256], s_eq[64];
   __shared__ uint16_t s0_ne[256], s0_eq[64];
   __shared__ uint32_t s_di[32];
@@ -200,7 +286,7 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
   int qn = 0;
   unsigned long long moves = 0;
   const bool telem = a.telem != nullptr;
-  unsigned long long t_start = telem ? clock64() : 0, t_hall = 0, n_flush = 0, n_tested = 0;
+  unsigned long long t_start = telem ? clock64() : 0, t_hall = 0, n_flush = 0, n_tested = 0, n_recheck = 0, n_bad = 0;
 
   while (true) {
     // ---- 1. idle lanes take the next items (one atomic per wave)
@@ -341,7 +427,20 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
       Leaf e;
       if (have) e = q[lane];
       else { e.X = e.Y = e.ell = 0; e.item = e.leaf = 0; }
-      const bool pass = hall_pass(e.X, e.Y, have, L, a.lim, a.hcos, a.hsin, s_order, s_rej);
+      const unsigned hv = hall_mfma(q, L, a.hall_nkb, (const u32x4_t *)a.hall_frag, a.hall_hi, a.hall_lo, lane);
+      const bool sure_rej = (hv & 1u) != 0, unsure = !sure_rej && (hv & 2u);
+      bool pass = have && !sure_rej && !unsure;
+      // uncertain leaves (and, in verify mode, every leaf) take the FP64 test
+      const bool re = have && (unsure || a.hall_verify);
+      const unsigned long long rm = __ballot(re);
+      if (rm) {
+        const bool full = hall_pass(e.X, e.Y, re, L, a.lim, a.hcos, a.hsin, s_order, s_rej);
+        if (telem) {
+          n_recheck += __popcll(__ballot(have && unsure));
+          if (a.hall_verify) n_bad += __popcll(__ballot(have && !unsure && full != pass));
+        }
+        if (re) pass = full;
+      }
       if (pass) atomicAdd(&a.item_cands[e.item], 1u);
       const unsigned long long pm = __ballot(pass);
       if (pm) {
@@ -382,6 +481,8 @@ __global__ __launch_bounds__(256) void stream_kernel(StreamArgs a) {
     atomicAdd(&a.telem[1], (unsigned long long)(clock64() - t_start));
     atomicAdd(&a.telem[2], n_flush);
     atomicAdd(&a.telem[3], n_tested);
+    atomicAdd(&a.telem[4], n_recheck);
+    atomicAdd(&a.telem[5], n_bad);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < 200; i += blockDim.x)
