@@ -1,24 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark: cell searches per second at n = 44 on one MI355X, batched vs plain.
+"""Benchmark: n = 44 cell searches per second on one MI355X, served vs plain GPU.
 
-Workload (BS(45,44), the reference's n = 44 production setting): every signature class at
-n = 44 (or --classes), mod-6 cells with Thm 2.11b/2.12, orbit canon, flat order; cells drawn
-from each class's live list by a seeded splitmix64. One cell search = the front-only unit
-of the reference arm: stream the cell's C,D candidates in DFS order (at most --node-budget
-DFS nodes and --cand-cap candidates), stable-sort them by flatness, complete the top --top-k
-with --ab-budget A,B nodes each under the cell's profile rows. Inputs are synthetic in the
-sense that cells are sampled; every computation is the real one.
+Workload (BS(45,44), the reference's n = 44 production setting): the 12 signature classes
+at n = 44 (or --classes), mod-6 cells with Thm 2.11b/2.12 (forced at n >= 36), orbit canon,
+flat order; cells drawn from each class's live list by a seeded splitmix64. One cell
+search is the front-only unit of the reference arm: stream the cell's C,D candidates in
+DFS order (at most --node-budget DFS nodes, a serving policy; the reference caps cells by
+candidates or wall time only), stable-sort them by flatness, complete the top --top-k with
+--ab-budget A,B nodes each under the cell's profile rows. Inputs are synthetic in the sense
+that cells are sampled; every computation is the real one, results are exact.
 
-A step = one pass of the hot path over the batch of sampled cells:
-  batched: all cells of a class in one persistent stream launch + one completer launch
-  plain:   one cell per launch (the baseline single-cell GPU path)
-`value` is the batched cells/s. The plain rate, candidate rates, per-cell latency p50/p99,
-the stream kernel's roofline and a CPU baseline (the oracle port on the host cores, on a
-bounded sample) are reported beside it on the same JSON line.
+A step = one pass of the hot path over the batch of sampled cells, served: every cell is
+submitted to the continuous-batching server (one worker and HIP stream per class, requests
+of a class admitted together) and the step ends when every result is back. `value` is
+served cells/s. Beside it on the same JSON line:
+  plain    one cell per search call (the single-cell GPU path), cells/s and latency
+  batched  one persistent launch per class (the kernel-level view; roofline source)
+  latency  open-loop Poisson arrivals at --load x the served rate: p50 / p99 cell latency
+  roofline the stream kernel's INT32-VALU rate on counted DFS nodes, against peak
+  cpu      the oracle port on the host cores on a bounded sample of the same cells
 """
 import argparse
 import json
 import os
+import random
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -31,14 +36,15 @@ sys.path.insert(0, str(ROOT))
 
 import bs45_quantum_explorer_amd as wz  # noqa: E402
 
-# MI355X figures used for the roofline (published peaks; INT32 VALU is derived from the
-# published CU count and clock: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz).
-HBM_PEAK_GBS = 8000.0
+# MI355X figures used for the roofline. INT32 VALU peak from the published CU count and
+# clock: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (one 32-bit op per lane per cycle).
 INT_VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
-OPS_PER_CD_NODE = 84          # per counted C,D DFS node (rec() entry), m = 6
+OPS_PER_CD_NODE = 84          # per counted C,D DFS node (rec() entry) at m = 6
 
 
 def classes(n):
+    """Signature classes (a <= b, c <= d) with a^2+b^2+c^2+d^2 = 4n+2 and the parities of
+    lengths n+1 (A,B) and n (C,D)."""
     out = []
     for a in range(n + 2):
         for b in range(a, n + 2):
@@ -71,15 +77,60 @@ def sample_cells(live, k, seed):
     return sorted(live[:k])
 
 
-_PROBS = []
-
-
 def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
-def percentile(v, q):
-    return float(np.percentile(np.asarray(v, dtype=np.float64), q)) if len(v) else 0.0
+def pct(v, q):
+    return round(1e3 * float(np.percentile(np.asarray(v, dtype=np.float64), q)), 3) if len(v) else None
+
+
+def key(r):
+    s = r.stream
+    return (s.cand, s.dfs_nodes, r.completed, r.hit, r.hit_rank, r.ab_nodes, r.clean_no, r.aborted, r.A, r.B)
+
+
+def served_pass(server, reqs, timeout_s=600):
+    """Submit every (problem, cell) request, wait for all results; returns results by request
+    index and the wall time."""
+    t0 = time.perf_counter()
+    tick = {}
+    for i, (pi, c) in enumerate(reqs):
+        tick[server.submit(pi, c)] = i
+    got = [None] * len(reqs)
+    n = 0
+    deadline = time.monotonic() + timeout_s
+    while n < len(reqs):
+        if time.monotonic() > deadline:
+            raise RuntimeError(f'served pass: {n} of {len(reqs)} results after {timeout_s} s')
+        for r in server.poll(max_results=4096, min_results=1, timeout_ms=2000):
+            if r.status != 0:
+                raise RuntimeError(f'served request failed with status {r.status}')
+            got[tick[r.ticket]] = r
+            n += 1
+    return got, time.perf_counter() - t0
+
+
+def poisson_latency(server, reqs, rate, count, seed):
+    """Open-loop Poisson arrivals at `rate` requests/s; latency = finish - submit."""
+    rng = random.Random(seed)
+    order = [reqs[rng.randrange(len(reqs))] for _ in range(count)]
+    t0 = time.perf_counter()
+    next_t = 0.0
+    lat, pending, done = [], set(), 0
+    i = 0
+    while done < count:
+        now = time.perf_counter() - t0
+        while i < count and next_t <= now:
+            pending.add(server.submit(*order[i]))
+            i += 1
+            next_t += rng.expovariate(rate)
+        wait_ms = int(max(0.0, min(50.0, (next_t - now) * 1e3))) if i < count else 2000
+        for r in server.poll(max_results=4096, min_results=1, timeout_ms=wait_ms):
+            pending.discard(r.ticket)
+            lat.append(r.finish_s - r.submit_s)
+            done += 1
+    return lat, time.perf_counter() - t0
 
 
 def main():
@@ -90,98 +141,93 @@ def main():
     ap.add_argument('--n', type=int, default=44)
     ap.add_argument('--classes', type=int, default=12, help='how many of the n=44 classes to use')
     ap.add_argument('--cells-per-class', type=int, default=256)
-    ap.add_argument('--plain-cells', type=int, default=32, help='cells timed through the plain path')
-    ap.add_argument('--node-budget', type=int, default=1 << 26)
+    ap.add_argument('--node-budget', type=int, default=1 << 22)
     ap.add_argument('--cand-cap', type=int, default=0)
     ap.add_argument('--top-k', type=int, default=32)
-    ap.add_argument('--ab-budget', type=int, default=1 << 20)
+    ap.add_argument('--ab-budget', type=int, default=1 << 24)
+    ap.add_argument('--plain-cells', type=int, default=2, help='cells per class timed one per call')
+    ap.add_argument('--latency-requests', type=int, default=1500)
+    ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')
+    ap.add_argument('--max-batch', type=int, default=4096)
     ap.add_argument('--seed', type=int, default=44)
-    ap.add_argument('--cpu-cells', type=int, default=6, help='cells per class in the CPU baseline sample')
+    ap.add_argument('--cpu-cells', type=int, default=4, help='cells per class in the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({'error': 'this engine runs on one GPU; multi-GPU sharding is design-only '
-                                   '(cells shard by pi mod G with a final min over (rank, idx))'}))
+        print(json.dumps({'error': 'this build runs on one GPU; cells shard by cell index mod G across GPUs '
+                                   'with a final min over (rank, idx), which is not built here'}))
         sys.exit(2)
 
     t_setup = time.time()
     cls = classes(a.n)[:a.classes]
-    probs, cells = _PROBS, []
+    probs, cells = [], []
     for i, sig in enumerate(cls):
         P = wz.Problem(a.n, sig, canon=1, order=1)
-        live = P.live_cells()
         probs.append(P)
-        cells.append(sample_cells(live, a.cells_per_class, a.seed * 1000 + i))
+        cells.append(sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000 + i))
+        log(f'setup class {i + 1}/{len(cls)} {sig}: {P.num_cells()} cells')
     setup_s = time.time() - t_setup
-    total_cells = sum(len(c) for c in cells)
+    reqs = [(pi, c) for pi, cl in enumerate(cells) for c in cl]
+    total_cells = len(reqs)
     log(f'setup: {len(cls)} classes, {total_cells} cells, {setup_s:.1f} s')
+    search = dict(cand_cap=a.cand_cap, node_budget=a.node_budget)
 
-    def step_batched():
-        res = []
-        st_ms = cp_ms = 0.0
-        st_units = 0
-        for P, cl in zip(probs, cells):
-            r = P.search(cl, a.top_k, a.ab_budget, cand_cap=a.cand_cap, node_budget=a.node_budget, batched=True)
-            ms0, u0 = P.last_kernel(0)
-            ms1, _ = P.last_kernel(1)
-            st_ms += ms0
-            cp_ms += ms1
-            st_units += u0
-            res.append(r)
-        return res, st_ms, cp_ms, st_units
+    # ---- served (the measured value)
+    server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=a.max_batch, **search)
+    try:
+        for w in range(a.warmup):
+            _, dt = served_pass(server, reqs)
+            log(f'served warmup {w + 1}/{a.warmup}: {dt * 1e3:.1f} ms')
+        times, served = [], None
+        for s in range(a.steps):
+            served, dt = served_pass(server, reqs)
+            times.append(dt)
+            log(f'served step {s + 1}/{a.steps}: {dt * 1e3:.1f} ms')
+        step_s = float(np.mean(times))
+        served_rate = total_cells / step_s
+        # ---- open-loop latency at a fraction of the served rate
+        rate = max(1.0, a.load * served_rate)
+        lat, lat_wall = poisson_latency(server, reqs, rate, a.latency_requests, a.seed)
+        log(f'latency: {len(lat)} Poisson requests at {rate:.1f}/s in {lat_wall:.1f} s')
+    finally:
+        server.close()
 
-    for w in range(a.warmup):
-        step_batched()
-        log(f'warmup step {w + 1}/{a.warmup} done')
-    times, kern = [], []
-    results = None
-    for _ in range(a.steps):
+    # ---- batched: one persistent launch per class (kernel timing for the roofline)
+    kern_ms, kern_units, batched_s = [], [], 0.0
+    for P, cl in zip(probs, cells):
         t0 = time.perf_counter()
-        results, st_ms, cp_ms, st_units = step_batched()
-        times.append(time.perf_counter() - t0)
-        kern.append((st_ms, cp_ms, st_units))
-        log(f'step {len(times)}/{a.steps}: {times[-1] * 1e3:.1f} ms (stream kernels {st_ms:.1f} ms, '
-            f'completer {cp_ms:.1f} ms)')
-    ms_per_step = 1000.0 * float(np.mean(times))
-    cells_per_s = total_cells / float(np.mean(times))
-    streamed = sum(r.stream.cand for rr in results for r in rr)
-    completed = sum(r.completed for rr in results for r in rr)
-    hits = sum(r.hit for rr in results for r in rr)
-    dfs_nodes = sum(r.stream.dfs_nodes for rr in results for r in rr)
-    ab_nodes = sum(r.ab_nodes for rr in results for r in rr)
+        P.search(cl, a.top_k, a.ab_budget, batched=True, **search)
+        batched_s += time.perf_counter() - t0
+        ms, units = P.last_kernel(0)
+        kern_ms.append(ms)
+        kern_units.append(units)
+    log(f'batched: {len(probs)} launches, {batched_s * 1e3:.1f} ms')
 
-    # plain path: one cell per launch, same cells (first plain-cells of each class, round-robin)
-    plain_lat, plain_res = [], {}
-    per_class = max(1, a.plain_cells // max(1, len(probs)))
-    t0 = time.perf_counter()
-    nplain = 0
+    # ---- plain: one cell per call
+    plain_lat, mismatch, nplain = [], 0, 0
     for ci, (P, cl) in enumerate(zip(probs, cells)):
-        for pi in cl[:per_class]:
-            t1 = time.perf_counter()
-            r = P.search([pi], a.top_k, a.ab_budget, cand_cap=a.cand_cap, node_budget=a.node_budget, batched=False)
-            plain_lat.append(time.perf_counter() - t1)
-            plain_res[(ci, pi)] = r[0]
+        for j, c in enumerate(cl[:a.plain_cells]):
+            t0 = time.perf_counter()
+            r = P.search([c], a.top_k, a.ab_budget, batched=False, **search)[0]
+            plain_lat.append(time.perf_counter() - t0)
             nplain += 1
-        log(f'plain: class {ci + 1}/{len(probs)} done ({nplain} cells)')
-    plain_s = time.perf_counter() - t0
-    plain_cells_per_s = nplain / plain_s if plain_s > 0 else 0.0
-    # the batched and plain paths must agree exactly on every cell they share
-    mismatch = 0
-    for ci, (P, cl) in enumerate(zip(probs, cells)):
-        for j, pi in enumerate(cl):
-            if (ci, pi) in plain_res:
-                x, y = plain_res[(ci, pi)], results[ci][j]
-                if (x.stream.cand, x.stream.dfs_nodes, x.completed, x.hit, x.ab_nodes, x.clean_no, x.aborted) != \
-                   (y.stream.cand, y.stream.dfs_nodes, y.completed, y.hit, y.ab_nodes, y.clean_no, y.aborted):
-                    mismatch += 1
+            if key(r) != key(served[reqs.index((ci, c))].result):
+                mismatch += 1
+    plain_rate = nplain / sum(plain_lat) if plain_lat else 0.0
+    log(f'plain: {nplain} cells, {plain_rate:.2f} cells/s, mismatches vs served {mismatch}')
 
-    st_ms = float(np.mean([k[0] for k in kern]))
-    cp_ms = float(np.mean([k[1] for k in kern]))
-    st_units = float(np.mean([k[2] for k in kern]))
-    nlaunch = len(probs)
-    avg_ms = st_ms / nlaunch
-    achieved_tops = OPS_PER_CD_NODE * (st_units / nlaunch) / (avg_ms / 1e3) / 1e12 if avg_ms > 0 else 0.0
+    res = [r.result for r in served]
+    streamed = sum(r.stream.cand for r in res)
+    completed = sum(r.completed for r in res)
+    dfs_nodes = sum(r.stream.dfs_nodes for r in res)
+    ab_nodes = sum(r.ab_nodes for r in res)
+    hits = sum(r.hit for r in res)
+
+    # roofline of the dominant kernel (the stream kernel), from the batched launches
+    avg_ms = float(np.mean(kern_ms))
+    units_per_launch = float(np.mean(kern_units))
+    achieved = OPS_PER_CD_NODE * units_per_launch / (avg_ms / 1e3) / 1e12 if avg_ms > 0 else 0.0
     traffic = None
     tf = ROOT / 'profiles' / 'stream_kernel_traffic.json'
     if tf.exists():
@@ -189,45 +235,46 @@ def main():
             traffic = json.loads(tf.read_text()).get('bytes_per_launch')
         except ValueError:
             traffic = None
-    roofline = {'kernel': 'wzfh::stream_kernel', 'bound': 'valu', 'achieved': round(achieved_tops, 3),
-                'peak': round(INT_VALU_PEAK_TOPS, 2), 'unit': 'TOP/s (INT32 VALU)',
-                'frac': round(achieved_tops / INT_VALU_PEAK_TOPS, 4), 'traffic': traffic,
-                'per_unit': f'{OPS_PER_CD_NODE} INT ops per counted C,D DFS node',
-                'units_per_launch': int(st_units / nlaunch), 'avg_launch_ms': round(avg_ms, 3),
-                'completer_ms_per_step': round(cp_ms, 3)}
+    roofline = {'bound': 'valu', 'achieved': round(achieved, 3), 'peak': round(INT_VALU_PEAK_TOPS, 2),
+                'unit': 'TOP/s (INT32 VALU)', 'frac': round(achieved / INT_VALU_PEAK_TOPS, 4), 'traffic': traffic,
+                'kernel': 'wzfh::stream_kernel',
+                'per_unit': f'{OPS_PER_CD_NODE} INT32 ops per counted C,D DFS node (rec() entry)',
+                'units_per_launch': int(units_per_launch), 'avg_launch_ms': round(avg_ms, 3)}
 
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline(a, cls, cells)
+    cpu = None if a.no_cpu else cpu_baseline(a, cls, probs, cells)
 
     line = {
         'metric': 'cells/sec & C,D cands/sec at n=44 (served vs plain GPU); p50/p99 cell latency',
-        'value': round(cells_per_s, 3), 'unit': 'cells/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
-        'ms_per_step': round(ms_per_step, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'int32+f64', 'data': 'synthetic (seeded cell sample of the real n=44 cell lists)',
-        'config': {'workload': f'BS(45,44) front-only cell search, {len(cls)} classes x {a.cells_per_class} '
+        'value': round(served_rate, 3), 'unit': 'cells/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
+        'ms_per_step': round(1e3 * step_s, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'int32 (DFS) + fp16x2/fp32 MFMA with fp64 recheck (leaf test)',
+        'data': 'synthetic: seeded splitmix64 sample of the real n=44 cell lists',
+        'config': {'workload': f'BS(45,44) front-only cell search: {len(cls)} classes x {a.cells_per_class} '
                                f'seeded cells, node budget {a.node_budget}, cand cap {a.cand_cap}, '
-                               f'top-{a.top_k}, A,B budget {a.ab_budget}',
+                               f'top-{a.top_k}, A,B budget {a.ab_budget}, served',
                    'n': a.n, 'classes': len(cls), 'cells_per_step': total_cells, 'node_budget': a.node_budget,
                    'cand_cap': a.cand_cap, 'top_k': a.top_k, 'ab_budget': a.ab_budget, 'seed': a.seed,
-                   'parallelism': 'single GPU, persistent work-stealing launches'},
-        'batched_cells_per_s': round(cells_per_s, 3),
-        'plain_cells_per_s': round(plain_cells_per_s, 3),
-        'batched_over_plain': round(cells_per_s / plain_cells_per_s, 2) if plain_cells_per_s else None,
-        'cands_streamed_per_s': round(streamed / float(np.mean(times)), 1),
-        'cands_completed_per_s': round(completed / float(np.mean(times)), 1),
-        'plain_latency_ms': {'p50': round(1e3 * percentile(plain_lat, 50), 3),
-                             'p99': round(1e3 * percentile(plain_lat, 99), 3)},
-        'batched_latency_ms': {'p50': round(ms_per_step / max(1, len(probs)), 3),
-                               'p99': round(1e3 * max(times), 3)},
+                   'parallelism': 'single GPU: continuous batching per class, persistent work-donating launches'},
+        'served_cells_per_s': round(served_rate, 3),
+        'plain_cells_per_s': round(plain_rate, 3),
+        'served_over_plain': round(served_rate / plain_rate, 2) if plain_rate else None,
+        'batched_cells_per_s': round(total_cells / batched_s, 3) if batched_s else None,
+        'cands_streamed_per_s': round(streamed / step_s, 1),
+        'cands_completed_per_s': round(completed / step_s, 1),
+        'dfs_nodes_per_s': round(dfs_nodes / step_s, 1),
+        'served_latency_ms': {'p50': pct(lat, 50), 'p99': pct(lat, 99), 'offered_per_s': round(rate, 2),
+                              'requests': len(lat)},
+        'plain_latency_ms': {'p50': pct(plain_lat, 50), 'p99': pct(plain_lat, 99), 'cells': nplain},
         'hits': hits, 'dfs_nodes_per_step': dfs_nodes, 'ab_nodes_per_step': ab_nodes,
-        'plain_vs_batched_mismatch': mismatch, 'setup_s': round(setup_s, 1),
+        'plain_vs_served_mismatch': mismatch, 'setup_s': round(setup_s, 1),
         'roofline': roofline, 'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)
+    for P in probs:
+        P.close()
 
 
-def cpu_baseline(a, cls, cells):
+def cpu_baseline(a, cls, probs, cells):
     """The oracle port (oracle/wz_oracle.cpp) on the host cores: the same cell unit on a
     bounded sample (the first --cpu-cells cells of each class), one cell per thread."""
     sys.path.insert(0, str(ROOT / 'tests'))
@@ -236,34 +283,34 @@ def cpu_baseline(a, cls, cells):
     except Exception as e:  # oracle not built: no baseline
         return {'error': f'oracle unavailable: {e}'}
     threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-    work = []
+    work, oprobs = [], []
     t_setup = time.time()
-    oprobs = []
-    for P_gpu, sig, cl in zip(_PROBS, cls, cells):
+    for P_gpu, sig, cl in zip(probs, cls, cells):
         pick = cl[:a.cpu_cells]
         P = OracleProblem(a.n, sig, canon=1, order=1, cells=[P_gpu.cell(pi)[:2] for pi in pick])
         oprobs.append(P)
-        for j in range(len(pick)):
-            work.append((P, j))
+        work += [(P, j) for j in range(len(pick))]
     setup = time.time() - t_setup
     cap = a.cand_cap if a.cand_cap > 0 else (1 << 22)
 
     def one(item):
-        P, pi = item
-        got, v, nd, idx, st = P.front_budget(pi, cap, a.node_budget, a.top_k, a.ab_budget)
-        return got
+        P, j = item
+        got, v, nd, idx, st = P.front_budget(j, cap, a.node_budget, a.top_k, a.ab_budget)
+        return got, st.dfs_nodes
+
     t0 = time.perf_counter()
+    cands = nodes = 0
     with ThreadPoolExecutor(threads) as ex:
-        cands = 0
-        for k, c in enumerate(ex.map(one, work)):
+        for k, (c, nd) in enumerate(ex.map(one, work)):
             cands += c
+            nodes += nd
             if (k + 1) % threads == 0:
                 log(f'cpu baseline: {k + 1}/{len(work)} cells')
     dt = time.perf_counter() - t0
     return {'value': round(len(work) / dt, 4), 'unit': 'cells/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{len(work)} cells ({a.cpu_cells} per class) of the same workload, same unit',
-            'cands_per_s': round(cands / dt, 1), 'seconds': round(dt, 2), 'setup_s': round(setup, 1),
-            'cpu_model': _cpu_model()}
+            'sample': f'{len(work)} cells ({a.cpu_cells} per class) of the same workload and unit',
+            'cands_per_s': round(cands / dt, 1), 'dfs_nodes_per_s': round(nodes / dt, 1), 'seconds': round(dt, 2),
+            'setup_s': round(setup, 1), 'cpu_model': _cpu_model()}
 
 
 def _cpu_model():

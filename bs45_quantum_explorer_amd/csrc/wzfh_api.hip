@@ -21,9 +21,8 @@
 namespace wzfh {
 
 hipError_t launch_stream(const StreamArgs &a, int blocks, hipStream_t st);
-hipError_t launch_stream_finalize(const StreamArgs &a, uint64_t *noff, uint64_t *loff, uint64_t *cell_tot,
-                                  unsigned long long ncands, KeptCand *kept, unsigned long long *nkept,
-                                  hipStream_t st);
+hipError_t launch_stream_finalize(const StreamArgs &a, uint64_t *cell_tot, unsigned long long ncands, KeptCand *kept,
+                                  unsigned long long *nkept, hipStream_t st);
 hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, hipStream_t st);
 
 static thread_local std::string g_err;
@@ -67,13 +66,16 @@ struct DeviceState {
   std::vector<unsigned long long> angle_hist;
   std::vector<uint8_t> kab_host;
   // per-call buffers
-  DBuf cell_res, cell_rootok, cut, item_nodes, item_leaves, item_cands, cands, ctr;
-  DBuf item_done, cell_lock, cell_pp, cell_pn, cell_pc, noff, loff, cell_tot, kept;
+  DBuf cell_res, cell_rootok, cands, ctr, cell_tot, kept;
+  // work-item records (static items, then donated ones) and per-cell walk state
+  DBuf it_nodes, it_noff, it_loff, it_coff, it_leaves, it_cands, it_state, it_next, it_key, it_meta, it_cell, it_tail;
+  DBuf dq, cell_lock, cell_crossed, cell_wp, cell_pn, cell_pl, cell_pc, cell_open, cell_next, cell_sp;
+  uint32_t epoch = 0;              // launch epoch tagging record states
   DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
-  unsigned long long last_telem[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long last_telem[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double hall_eps = 0.0;           // margin of the matrix-core leaf test
   // the last stream result, reused when the caller asks again for the same cells and
   // options (count first, then fetch) so the kernel runs once
@@ -291,9 +293,11 @@ void device_free(wzfh_problem &P) {
   DeviceState *S = P.dev;
   if (!S) return;
   for (DBuf *b : {&S->telem, &S->hall_frag, &S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->posidx, &S->rows, &S->kab,
-                  &S->cell_res, &S->cell_rootok, &S->cut, &S->item_nodes, &S->item_leaves, &S->item_cands,
-                  &S->cands, &S->ctr, &S->item_done, &S->cell_lock, &S->cell_pp, &S->cell_pn, &S->cell_pc,
-                  &S->noff, &S->loff, &S->cell_tot, &S->kept, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
+                  &S->cell_res, &S->cell_rootok, &S->cands, &S->ctr, &S->cell_tot, &S->kept,
+                  &S->it_nodes, &S->it_noff, &S->it_loff, &S->it_coff, &S->it_leaves, &S->it_cands, &S->it_state,
+                  &S->it_next, &S->it_key, &S->it_meta, &S->it_cell, &S->it_tail, &S->dq, &S->cell_lock,
+                  &S->cell_crossed, &S->cell_wp, &S->cell_pn, &S->cell_pl, &S->cell_pc, &S->cell_open,
+                  &S->cell_next, &S->cell_sp, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
                   &S->c_B, &S->c_scratch})
     b->release();
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
@@ -313,11 +317,10 @@ static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncel
   const int half = P.n / 2;
   int d0 = o.split_depth;
   if (d0 <= 0) {
-    // capped cells are DFS prefixes: split deep so a prefix spans many subtrees; whole
-    // cells: enough paths per cell to fill the GPU without huge per-item arrays
-    const bool capped = o.cand_cap > 0 || o.node_budget > 0;
-    if (capped) d0 = P.n >= 30 ? 6 : 4;
-    else d0 = ncells >= 4096 ? 2 : (ncells >= 256 ? 3 : 4);
+    // static items only seed the launch: running items donate their untried subtrees to
+    // idle lanes, so a few hundred paths per cell (fewer for big batches) suffice
+    d0 = ncells >= 256 ? 2 : 3;
+    d0 = std::max(d0, half - 21);   // an item's digit word holds 21 levels below its base
   }
   return std::max(1, std::min(d0, std::max(1, half)));
 }
@@ -328,8 +331,13 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
   WZ_HIP(hipSetDevice(S->device));   // callers may be any host thread (serving workers)
   StreamArgs a = S->base;
   const int d0 = auto_split(P, o, ncells);
+  if (P.n / 2 - d0 > 21) { set_error("split depth too shallow: an item spans at most 21 levels"); return WZFH_EINVAL; }
+  if (a.nch0 * (double)(1u << std::min(30, 3 * (d0 - 1))) * ncells > 2.0e9) {
+    set_error("split depth too deep for this many cells");
+    return WZFH_EINVAL;
+  }
   const int npaths = a.nch0 * (1 << (3 * (d0 - 1)));
-  int window = o.window > 0 ? o.window : ((o.cand_cap > 0 || o.node_budget > 0) ? std::min(npaths, 64) : npaths);
+  int window = o.window > 0 ? o.window : ((o.cand_cap > 0 || o.node_budget > 0) ? std::min(npaths, 16) : npaths);
   window = std::max(1, std::min(window, npaths));
   a.d0 = d0; a.npaths = npaths; a.window = window; a.ncells = ncells;
   a.node_budget = o.node_budget; a.cand_cap = o.cand_cap;
@@ -349,12 +357,19 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     }
     res[2 * i] = rx; res[2 * i + 1] = ry; rootok[i] = ok;
   }
-  const size_t nitems = (size_t)ncells * npaths;
-  if (S->cell_res.ensure(res.size() * 8) || S->cell_rootok.ensure(ncells * 4) || S->cut.ensure(ncells * 4) ||
-      S->item_nodes.ensure(nitems * 8) || S->item_leaves.ensure(nitems * 4) || S->item_cands.ensure(nitems * 4) ||
-      S->item_done.ensure(nitems * 4) || S->cell_lock.ensure(ncells * 4) || S->cell_pp.ensure(ncells * 4) ||
-      S->cell_pn.ensure(ncells * 8) || S->cell_pc.ensure(ncells * 8) || S->noff.ensure(nitems * 8) ||
-      S->loff.ensure(nitems * 8) || S->cell_tot.ensure(ncells * 24) || S->ctr.ensure(64))
+  const size_t nstatic = (size_t)ncells * npaths;
+  const size_t ndonate = std::max<size_t>(1u << 22, 64 * (size_t)ncells);
+  const size_t nrec = nstatic + ndonate;
+  if (nrec >= 0xFFFFFFF0ull) { set_error("too many work items for one launch"); return WZFH_ECAPACITY; }
+  if (S->cell_res.ensure(res.size() * 8) || S->cell_rootok.ensure(ncells * 4) || S->cell_tot.ensure(ncells * 24) ||
+      S->ctr.ensure(64) || S->dq.ensure(64) || S->it_nodes.ensure(nrec * 8) || S->it_noff.ensure(nrec * 8) ||
+      S->it_loff.ensure(nrec * 8) || S->it_coff.ensure(nrec * 8) || S->it_leaves.ensure(nrec * 4) ||
+      S->it_cands.ensure(nrec * 4) || S->it_state.ensure(nrec * 4) || S->it_next.ensure(nrec * 4) ||
+      S->it_key.ensure(nrec * 8) || S->it_meta.ensure(nrec * 4) || S->it_cell.ensure(nrec * 4) ||
+      S->it_tail.ensure(nrec * 4) || S->cell_lock.ensure(ncells * 4) || S->cell_crossed.ensure(ncells * 4) ||
+      S->cell_wp.ensure(ncells * 4) || S->cell_pn.ensure(ncells * 8) || S->cell_pl.ensure(ncells * 8) ||
+      S->cell_pc.ensure(ncells * 8) || S->cell_open.ensure(ncells * 4) || S->cell_next.ensure(ncells * 4) ||
+      S->cell_sp.ensure(ncells * 4))
     return WZFH_ENOMEM;
   // candidate buffer: what the caps allow, at most 4M records up front, grown on overflow
   unsigned long long want = o.cand_cap > 0 ? (unsigned long long)ncells * (unsigned long long)(o.cand_cap + 512)
@@ -366,15 +381,6 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     WZ_HIP(hipMemcpyAsync(S->cell_res.p, res.data(), res.size() * 8, hipMemcpyHostToDevice, S->stream));
     WZ_HIP(hipMemcpyAsync(S->cell_rootok.p, rootok.data(), ncells * 4, hipMemcpyHostToDevice, S->stream));
     WZ_HIP(hipMemcpyAsync(S->order.p, S->angle_order.data(), 200, hipMemcpyHostToDevice, S->stream));
-    WZ_HIP(hipMemsetAsync(S->cut.p, 0x7F, ncells * 4, S->stream));
-    WZ_HIP(hipMemsetAsync(S->item_nodes.p, 0, nitems * 8, S->stream));
-    WZ_HIP(hipMemsetAsync(S->item_leaves.p, 0, nitems * 4, S->stream));
-    WZ_HIP(hipMemsetAsync(S->item_cands.p, 0, nitems * 4, S->stream));
-    WZ_HIP(hipMemsetAsync(S->item_done.p, 0, nitems * 4, S->stream));
-    WZ_HIP(hipMemsetAsync(S->cell_lock.p, 0, ncells * 4, S->stream));
-    WZ_HIP(hipMemsetAsync(S->cell_pp.p, 0, ncells * 4, S->stream));
-    WZ_HIP(hipMemsetAsync(S->cell_pn.p, 0, ncells * 8, S->stream));
-    WZ_HIP(hipMemsetAsync(S->cell_pc.p, 0, ncells * 8, S->stream));
     WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
     WZ_HIP(hipMemsetAsync(S->rej.p, 0, 200 * sizeof(unsigned), S->stream));
     a.cell_res = S->cell_res.as<uint64_t>();
@@ -385,15 +391,35 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     a.angle_order = S->order.as<uint8_t>();
     a.work_ctr = S->ctr.as<unsigned long long>();
     a.ncands = S->ctr.as<unsigned long long>() + 1;
-    a.cut = S->cut.as<int>();
-    a.item_nodes = S->item_nodes.as<uint64_t>();
-    a.item_leaves = S->item_leaves.as<uint32_t>();
-    a.item_cands = S->item_cands.as<uint32_t>();
-    a.item_done = S->item_done.as<uint32_t>();
+    a.rec_cap = (uint32_t)nrec;
+    a.epoch = ++S->epoch;
+    if (S->epoch >= (1u << 29)) S->epoch = 0;   // tags use 30 bits; fresh buffers hold 0
+    a.it_nodes = S->it_nodes.as<uint64_t>();
+    a.it_noff = S->it_noff.as<uint64_t>();
+    a.it_loff = S->it_loff.as<uint64_t>();
+    a.it_coff = S->it_coff.as<uint64_t>();
+    a.it_leaves = S->it_leaves.as<uint32_t>();
+    a.it_cands = S->it_cands.as<uint32_t>();
+    a.it_state = S->it_state.as<uint32_t>();
+    a.it_next = S->it_next.as<uint32_t>();
+    a.it_key = S->it_key.as<uint64_t>();
+    a.it_meta = S->it_meta.as<uint32_t>();
+    a.it_cell = S->it_cell.as<uint32_t>();
+    a.it_tail = S->it_tail.as<uint32_t>();
+    a.dq = S->dq.as<unsigned long long>();
     a.cell_lock = S->cell_lock.as<int>();
-    a.cell_pp = S->cell_pp.as<int>();
+    a.cell_crossed = S->cell_crossed.as<int>();
+    a.cell_wp = S->cell_wp.as<uint32_t>();
     a.cell_pn = S->cell_pn.as<unsigned long long>();
+    a.cell_pl = S->cell_pl.as<unsigned long long>();
     a.cell_pc = S->cell_pc.as<unsigned long long>();
+    a.cell_open = S->cell_open.as<int>();
+    a.cell_next = S->cell_next.as<uint32_t>();
+    a.cell_sp = S->cell_sp.as<uint32_t>();
+    a.static_win = 4;
+    a.stall_ticks = 500000000ull;   // 5 s of the 100 MHz constant clock
+
+    if (const char *e = getenv("WZFH_STATIC_WINDOW")) a.static_win = (uint32_t)std::max(1, atoi(e));
     a.cands = S->cands.as<CandRec>();
     a.cand_capacity = cap;
     a.angle_rej = S->rej.as<unsigned>();
@@ -403,24 +429,40 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
       WZ_HIP(hipMemsetAsync(S->telem.p, 0, 64, S->stream));
       a.telem = S->telem.as<unsigned long long>();
     }
-    const unsigned long long work_items = (unsigned long long)((npaths + window - 1) / window) * ncells * window;
-    int blocks = (int)std::min<unsigned long long>((unsigned long long)S->num_cu * 8, (work_items + 255) / 256 + 1);
-    blocks = std::max(blocks, 1);
+    // persistent grid: as many waves as can be resident (4 per SIMD at the kernel's
+    // register budget); donation keeps them busy once the static items are gone
+    const int blocks = std::max(1, S->num_cu * 4);
+    // idle polling backoff: waves per cell, so a few cells are not probed by thousands of
+    // waves every iteration (the counters are device-scope atomics)
+    a.scan_max = std::max(8, std::min(1024, (blocks * 4) / std::max(1, ncells)));
+    a.don_moves = 1024;
+    a.don_moves_front = 64;
+    a.don_hops = 2;
+    a.don_low = (uint32_t)(blocks * 256 / 4);
+    if (const char *e = getenv("WZFH_DONATE_MOVES")) a.don_moves = a.don_moves_front = (uint32_t)std::max(1, atoi(e));
+    if (const char *e = getenv("WZFH_DONATE_HOPS")) a.don_hops = (uint32_t)std::min(15, std::max(0, atoi(e)));
+    a.don_any = (getenv("WZFH_DONATE_ANY") && atoi(getenv("WZFH_DONATE_ANY"))) ? 1 : 0;
+    if (const char *e = getenv("WZFH_DONATE_LOW")) a.don_low = (uint32_t)std::max(0, atoi(e));
+    a.don_max_level = std::min(21, std::max(0, a.half - 3));
     WZ_HIP(hipEventRecord(S->ev[0], S->stream));
     WZ_HIP(launch_stream(a, blocks, S->stream));
     WZ_HIP(hipEventRecord(S->ev[1], S->stream));
-    unsigned long long ctr[2];
+    unsigned long long ctr[2], dqh[6];
     WZ_HIP(hipMemcpyAsync(ctr, S->ctr.p, 16, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(dqh, S->dq.p, sizeof dqh, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipStreamSynchronize(S->stream));
+    if (dqh[5]) {
+      set_error("stream kernel made no progress for 5 s and stopped (internal error)");
+      return WZFH_EINTERNAL;
+    }
     float ms = 0;
     WZ_HIP(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
     S->last_ms[0] = ms;
-    if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 48, hipMemcpyDeviceToHost));
+    if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 64, hipMemcpyDeviceToHost));
     if (ctr[1] > cap) { cap = ctr[1] + (ctr[1] >> 2) + 1024; continue; }
     // ---- exact global order on the device; only the cell totals and kept candidates return
     unsigned long long *nkept = S->ctr.as<unsigned long long>() + 2;
-    WZ_HIP(launch_stream_finalize(a, S->noff.as<uint64_t>(), S->loff.as<uint64_t>(), S->cell_tot.as<uint64_t>(),
-                                  ctr[1], S->kept.as<KeptCand>(), nkept, S->stream));
+    WZ_HIP(launch_stream_finalize(a, S->cell_tot.as<uint64_t>(), ctr[1], S->kept.as<KeptCand>(), nkept, S->stream));
     unsigned long long nk = 0;
     std::vector<uint64_t> tot(3 * (size_t)ncells);
     std::vector<unsigned> rej(200);
@@ -428,6 +470,11 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     WZ_HIP(hipMemcpyAsync(tot.data(), S->cell_tot.p, tot.size() * 8, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipMemcpyAsync(rej.data(), S->rej.p, 200 * sizeof(unsigned), hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipStreamSynchronize(S->stream));
+    for (int cs = 0; cs < ncells; cs++)
+      if (tot[3 * cs] == kUnsetOff) {
+        set_error("stream: an uncut cell has an unfinished work item");
+        return WZFH_EINTERNAL;
+      }
     std::vector<KeptCand> kc(nk);
     if (nk) WZ_HIP(hipMemcpy(kc.data(), S->kept.p, nk * sizeof(KeptCand), hipMemcpyDeviceToHost));
     // refine the angle order for the next launch (decision-invariant)
@@ -715,8 +762,8 @@ int wzfh_npaf_check(int n, const int8_t *A, const int8_t *B, const int8_t *C, co
 }
 
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units) {
-  if (!p || !p->dev || kind < 0 || kind > 8) return WZFH_EINVAL;
-  if (kind == 8) {   // margin of the matrix-core leaf test
+  if (!p || !p->dev || kind < 0 || kind > 10) return WZFH_EINVAL;
+  if (kind == 10) {   // margin of the matrix-core leaf test
     if (ms) *ms = p->dev->hall_eps;
     if (units) *units = 0;
     return WZFH_OK;

@@ -64,6 +64,7 @@ struct wzfh_problem {
 namespace wzfh {
 // Host setup (wzfh_setup.cpp).
 int build_problem(wzfh_problem &P);
+int side_profiles3(const wzfh_config &c, int n, bool ab_side, std::vector<HostCell> &out);
 // Device side (wzfh_device.hip).
 int device_init(wzfh_problem &P);
 void device_free(wzfh_problem &P);

@@ -5,7 +5,7 @@
 namespace wzfh {
 
 // One streamed candidate: packed C,D (bit i set = element i is -1), the DFS node
-// index of its leaf inside its work item, the work item, and the leaf ordinal
+// index of its leaf inside its work item, the work item's record, and the leaf ordinal
 // inside the item (for exact leaf counts under a candidate cap).
 struct CandRec {
   uint64_t X, Y;
@@ -14,13 +14,16 @@ struct CandRec {
   uint32_t leaf;
 };
 
-// A candidate placed in its cell's global DFS order (after stream_finalize).
+// A candidate placed in its cell's global DFS order (after the stream's walk).
 struct KeptCand {
   uint64_t X, Y;
   uint64_t G;       // global DFS node index of the leaf inside its cell
   uint64_t leaf;    // global leaf index inside its cell
   uint64_t cell;    // cell slot of the launch
 };
+
+constexpr uint32_t kEndItem = 0xFFFFFFFFu;      // end of a cell's item list
+constexpr uint64_t kUnsetOff = ~0ull;           // item offset not known (yet)
 
 // C,D stream kernel arguments (count_pairs22 on the C,D side, wz_match.cpp:678-784).
 struct StreamArgs {
@@ -40,22 +43,44 @@ struct StreamArgs {
   const double *hcos, *hsin;   // [201][64]
   const uint8_t *angle_order;  // [200] angle j-1 in evaluation order
   long long node_budget, cand_cap;
-  unsigned long long *work_ctr;
-  int *cut;                    // [ncells] first path index known to be beyond the cut
-  uint64_t *item_nodes;        // [ncells * npaths]
-  uint32_t *item_leaves;       // [ncells * npaths]
-  uint32_t *item_cands;        // [ncells * npaths]
-  uint32_t *item_done;         // [ncells * npaths] set once the item's counts are final
-  int *cell_lock;              // [ncells] completed-prefix tracker (try-lock)
-  int *cell_pp;                // [ncells] items in the completed prefix
-  unsigned long long *cell_pn, *cell_pc;   // [ncells] its node and candidate sums
+  unsigned long long *work_ctr;   // static items handed out
+  // Work items. Records [0, ncells*npaths) are the static items (path p of cell slot cs
+  // at cs*npaths + p); records from there up to rec_cap are items donated by running
+  // lanes. Each cell's items form a list (it_next) in the reference's DFS order; record
+  // states carry the launch epoch: (epoch << 2) | 1 published donation, | 2 done,
+  // | 3 abandoned beyond the cell's cut.
+  uint32_t rec_cap, epoch;
+  uint64_t *it_nodes;             // nodes of the item's run (incl. its tail)
+  uint64_t *it_noff, *it_loff, *it_coff;   // node / leaf / candidate offsets in the cell
+  uint32_t *it_leaves, *it_cands, *it_state, *it_next;
+  uint64_t *it_key;               // donated: digits of the levels above its base
+  uint32_t *it_meta;              // donated: base level | lo << 8 | hi << 12 | hops << 16
+  uint32_t *it_cell, *it_tail;    // donated: cell slot; dead entries after its subtrees
+  unsigned long long *dq;         // [0] records allocated, [1] claimed, [2] items in hand,
+                                  // [3] capped launches: cells still handing out static items,
+                                  // [4] progress (items finished + donations), [5] stall flag
+  unsigned long long stall_ticks; // watchdog: constant-clock ticks without progress
+  uint32_t don_moves, don_low;    // donation period (DFS moves) and pending low-water
+  uint32_t don_moves_front;       // donation period of a capped cell's frontier item
+  uint32_t don_hops;              // capped: generations of frontier donations that donate again
+  int don_any;                    // 1: every item may donate (also under a cap)
+  int don_max_level;              // deepest level whose siblings are donated
+  int *cell_lock, *cell_crossed;  // walker try-lock; 1 once the cut is known
+  // capped launches hand out a cell's static items through its own cursor, at most
+  // static_win items ahead of the walk, so lanes work near each cell's frontier
+  int *cell_open;                 // 1 while the cell still has static items to hand out
+  uint32_t *cell_next, *cell_sp;  // next static item to hand out; static items walked past
+  uint32_t static_win;
+  int scan_max;                   // longest backoff (loop iterations) of idle polling
+  uint32_t *cell_wp;              // first item not yet walked
+  unsigned long long *cell_pn, *cell_pl, *cell_pc;   // walked node / leaf / candidate sums
   CandRec *cands;
   unsigned long long *ncands;
   unsigned long long cand_capacity;
   unsigned int *angle_rej;     // [200] first-rejecting-angle histogram (angle order tuning)
   // Leaf test on the matrix cores: A-operand fragments of the split angle table (see
   // hall_mfma in wzfh_stream.hip), the certain-reject / uncertain thresholds on the pair
-  // energy, the number of 16-position K blocks covering L, and a verify switch that also
+  // energy, the number of 32-position K blocks covering L, and a verify switch that also
   // runs the FP64 test on every leaf and counts disagreements with the certain decisions.
   const void *hall_frag;
   float hall_hi, hall_lo;
@@ -63,7 +88,8 @@ struct StreamArgs {
   int hall_verify;
   unsigned long long *telem;   // optional per-launch counters (null = off): [0] cycles in the
                                // leaf test, [1] total wave cycles, [2] flushes, [3] leaves tested,
-                               // [4] leaves rechecked in FP64, [5] disagreements (verify mode)
+                               // [4] leaves rechecked in FP64, [5] disagreements (verify mode),
+                               // [6] donations, [7] items donated
 };
 
 // A,B completer kernel arguments (fh_complete_ab, wz_match.cpp:1092-1341).

@@ -251,6 +251,60 @@ long long prof_score(const HostCell &p, int m) {
 
 }  // namespace
 
+// survive_profiles (wz_match.cpp:439-487) for one side of the join: this side's m = 3
+// profiles (px, py) that the complement side's norm sets admit, with the mod-6 tighten
+// (some lift to m = 6 passes eq 2.12 and, with 2.11b, cancels its residue
+// autocorrelations against the complement's achievable tuples), in px-major order.
+// ab_side: the A,B side (length n+1, sigs a, b; complement C,D), else the C,D side.
+int side_profiles3(const wzfh_config &c, int n, bool ab_side, std::vector<HostCell> &out) {
+  const int tgt = 4 * n + 2;
+  const bool t212 = c.thm212 != 0 || n >= 36, t211b = c.thm211b != 0 || n >= 36;
+  const int L = ab_side ? n + 1 : n, Lc = ab_side ? n : n + 1;
+  const int sx = ab_side ? c.a : c.c, sy = ab_side ? c.b : c.d;
+  const int cx = ab_side ? c.c : c.a, cy = ab_side ? c.d : c.b;
+  NormBits cmp3, cmp6;
+  cmp3.build(Lc, cx, cy, 3, tgt);
+  cmp6.build(Lc, cx, cy, 6, tgt);
+  AutoKeys cmp6a;
+  if (t211b) cmp6a.build(Lc, cx, cy, 6, n, t212, !ab_side);
+  auto PX3 = class_tuples(L, sx, 3), PY3 = class_tuples(L, sy, 3);
+  auto PX6 = class_tuples(L, sx, 6), PY6 = class_tuples(L, sy, 6);
+  Eq212 e3(3, n, ab_side), e6(6, n, ab_side);
+  int T[3];
+  for (auto &px : PX3) {
+    int npx = sqn(px);
+    if (npx > tgt) continue;
+    for (auto &py : PY3) {
+      int need = tgt - npx - sqn(py);
+      if (!cmp3.ok(need)) continue;
+      if (t212 && !e3.ok(px, py)) continue;
+      bool ok6 = false;
+      for (auto &px6 : PX6) {
+        if (reduce63(px6) != px) continue;
+        int npx6 = sqn(px6);
+        if (npx6 > tgt) continue;
+        for (auto &py6 : PY6) {
+          if (reduce63(py6) != py) continue;
+          int need6 = tgt - npx6 - sqn(py6);
+          if (need6 < 0) continue;
+          if (t212 && !e6.ok(px6, py6)) continue;
+          if (t211b) {
+            for (int s = 1; s <= 3; s++) T[s - 1] = -side_auto(px6, py6, s);
+            if (cmp6a.has(T, 3, need6)) { ok6 = true; break; }
+          } else if (cmp6.ok(need6)) { ok6 = true; break; }
+        }
+        if (ok6) break;
+      }
+      if (!ok6) continue;
+      HostCell h{};
+      for (int q = 0; q < 3; q++) { h.px[q] = px[q]; h.py[q] = py[q]; }
+      out.push_back(h);
+      if (out.size() >= 20000000) return WZFH_ETRUNC;
+    }
+  }
+  return WZFH_OK;
+}
+
 int build_problem(wzfh_problem &P) {
   const wzfh_config &c = P.cfg;
   const int n = P.n, L1 = P.L1, m = P.m, tgt = 4 * n + 2;
@@ -312,38 +366,8 @@ int build_problem(wzfh_problem &P) {
       cells.swap(keep);
     }
   } else {
-    // survive_profiles (mod-3 with the mod-6 tighten, wz_match.cpp:439-487)
-    auto PX3 = class_tuples(n, c.c, 3), PY3 = class_tuples(n, c.d, 3);
-    auto PX6 = class_tuples(n, c.c, 6), PY6 = class_tuples(n, c.d, 6);
-    Eq212 e3(3, n, false), e6(6, n, false);
-    int T[3];
-    for (auto &px : PX3) {
-      int npx = sqn(px);
-      if (npx > tgt) continue;
-      for (auto &py : PY3) {
-        int need = tgt - npx - sqn(py);
-        if (!ab3.ok(need)) continue;
-        if (t212 && !e3.ok(px, py)) continue;
-        bool ok6 = false;
-        for (auto &px6 : PX6) {
-          if (reduce63(px6) != px) continue;
-          int npx6 = sqn(px6);
-          if (npx6 > tgt) continue;
-          for (auto &py6 : PY6) {
-            if (reduce63(py6) != py) continue;
-            int need6 = tgt - npx6 - sqn(py6);
-            if (need6 < 0) continue;
-            if (t212 && !e6.ok(px6, py6)) continue;
-            if (t211b) {
-              for (int s = 1; s <= 3; s++) T[s - 1] = -side_auto(px6, py6, s);
-              if (ab6a.has(T, 3, need6)) { ok6 = true; break; }
-            } else if (ab6.ok(need6)) { ok6 = true; break; }
-          }
-          if (ok6) break;
-        }
-        if (ok6) push(px, py);
-      }
-    }
+    int rc = side_profiles3(c, n, false, cells);
+    if (rc != WZFH_OK) return rc;
   }
 
   // WZ_FH_PROF_ORDER 1/2 (wz_match.cpp:1633-1638): std::sort on the same input

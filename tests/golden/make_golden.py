@@ -21,6 +21,9 @@ Fixtures:
   firsthit_n29.json   the n = 29 known answer (WZ_FH_M6=1, default completer)
   solutions.json      the banked solution files (results/champions, results/reference)
                       and the quarantined invalid one (negative control)
+  joins.json          WZ_JOIN22 runs (single thread): both sides' profile counts, the C,D
+                      stream size, distinct keys, raw key hits, FOUND/EXHAUSTED and the
+                      printed solution (n = 6..15 every class, n = 17 and 19 samples)
 
 Usage: python tests/golden/make_golden.py [--only NAME] [--jobs 8]
 """
@@ -281,6 +284,44 @@ def gen_solutions(jobs):
     print(f'solutions.json: {len(recs)} files')
 
 
+JOIN_PROF = re.compile(r'^\[profiles\] A,B side surviving: (\d+)\s+C,D side surviving: (\d+)', re.M)
+JOIN_BUILD = re.compile(r'^\[join22v2\] C,D stream=([0-9.e+]+)\s+distinct keys=(\d+)', re.M)
+JOIN_HITS = re.compile(r'^\[join22v2\] phase 2 done: (\d+) raw key hits', re.M)
+
+
+def join_case(n, sig, settings):
+    cfg = {**dict(WZ_JOIN22=1, WZ_JOIN22_SLOTS_LOG2=22, OMP_NUM_THREADS=1), **settings}
+    rc, out = run(n, sig, cfg)
+    prof, build, hits = JOIN_PROF.search(out), JOIN_BUILD.search(out), JOIN_HITS.search(out)
+    if not prof or not build:
+        raise RuntimeError(f'join run n={n} sig={sig} {settings} rc={rc}: {out[-600:]}')
+    rec = dict(n=n, sig=list(sig), settings=cfg, rc=rc, ab_profiles=int(prof[1]), cd_profiles=int(prof[2]),
+               cd_stream=int(float(build[1])), cd_stream_exact='e' not in build[1],   # cout prints a double
+               distinct_keys=int(build[2]), hits=int(hits[1]) if hits else 0,
+               found='FOUND ***' in out, exhausted='JOIN22 EXHAUSTED' in out)
+    for name in 'ABCD':
+        mm = re.search(r'^' + name + r' = \{([^}]*)\}', out, re.M)
+        if mm:
+            rec[name] = [int(x) for x in mm[1].split(',')]
+    return rec
+
+
+def gen_join(jobs):
+    tasks = []
+    for n in range(6, 16):
+        for sig in classes(n):
+            tasks.append((n, sig, {}))
+            tasks.append((n, sig, {'WZ_THM211B': 1, 'WZ_THM212': 1}))
+    for n in (17, 19):
+        for sig in classes(n)[:3]:
+            tasks.append((n, sig, {'WZ_THM211B': 1, 'WZ_THM212': 1}))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: join_case(*t), tasks))
+    (HERE / 'joins.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'joins.json: {len(recs)} runs, {sum(r["found"] for r in recs)} found, '
+          f'{sum(r["cd_stream"] for r in recs)} C,D candidates, {sum(r["hits"] for r in recs)} hits')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--only', default='')
@@ -289,7 +330,7 @@ def main():
     if not REF.exists():
         sys.exit(f'{REF} missing: run make -C oracle ref (needs the reference tree)')
     gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29,
-                solutions=gen_solutions)
+                solutions=gen_solutions, join=gen_join)
     for name, g in gens.items():
         if not a.only or name in a.only.split(','):
             g(a.jobs)

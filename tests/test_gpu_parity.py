@@ -185,3 +185,29 @@ def test_stream_cap_and_budget_together_match_oracle():
                 assert (st[i].cand, st[i].dfs_nodes) == (got, ost.dfs_nodes), (cap, budget, pi)
     finally:
         P.close()
+
+
+@pytest.mark.parametrize('cap,budget', [(0, 0), (0, 37), (0, 4096), (3, 0), (3, 200)])
+def test_stream_forced_donation_matches_oracle(cap, budget, monkeypatch):
+    """Running items donate their untried subtrees every 2 DFS moves: item lists, offsets,
+    tails and the cut must still reproduce the oracle's stream exactly."""
+    from oracle_lib import OracleProblem
+    monkeypatch.setenv('WZFH_DONATE_MOVES', '2')
+    monkeypatch.setenv('WZFH_DONATE_LOW', '1000000')
+    monkeypatch.setenv('WZFH_DONATE_ANY', '1')
+    n, sig = 19, (2, 4, 3, 7)
+    P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1)
+    O = OracleProblem(n, sig, canon=1, order=1)
+    try:
+        cells = P.live_cells()[:24]
+        st, offs, cd = P.stream(cells, cand_cap=cap, node_budget=budget, split_depth=2, window=2)
+        for i, pi in enumerate(cells):
+            got, cands, ost = O.stream_budget(pi, cap, budget, 100000)
+            assert (st[i].cand, st[i].dfs_nodes) == (got, ost.dfs_nodes), (cap, budget, pi)
+            if cap == 0 and (budget == 0 or ost.dfs_nodes < budget):
+                assert st[i].leaves == ost.leaves
+            assert wz.dump_lines(n, cd[offs[i]:offs[i + 1]]) == [
+                ''.join(f'{int(v)} ' for v in c[0]) + '| ' + ''.join(f'{int(v)} ' for v in c[1]) + '\n'
+                for c in cands], (cap, budget, pi)
+    finally:
+        P.close()

@@ -51,3 +51,28 @@ def test_oracle_completer_matches_reference(n):
         assert (r, nodes) == (c['r'], c['nodes']), c['name']
         if r == 0:
             assert list(A) == c['A'] and list(B) == c['B']
+
+
+def test_oracle_node_budget_is_a_prefix_of_the_golden_stream():
+    """The node budget (a serving policy) truncates the reference recursion at rec() entry:
+    the budgeted stream must be a prefix of the golden-pinned full stream, grow monotonically
+    with the budget, report min(total nodes, budget) DFS nodes, and equal the full stream
+    (with exact leaves) once the budget covers the cell."""
+    rec = next(r for r in RUNS if r['n'] == 19 and r['canon'] == 1 and r['rev'] == 0)
+    P = OracleProblem(19, rec['sig'], canon=1, order=rec['order'])
+    for row in rec['cells'][::11]:
+        pi, total_nodes = row[0], row[4]
+        cnt, full, st = P.stream(pi, 0, 20000)
+        h = hashlib.sha1(''.join(dump_lines(full)).encode()).hexdigest()
+        assert [pi, cnt, st.leaves, st.hall_ok, st.dfs_nodes, h] == row   # the reference's stream
+        prev = 0
+        for budget in sorted({1, 2, 37, 500, 4096, total_nodes // 2, total_nodes - 1, total_nodes, total_nodes + 1}):
+            if budget < 1:
+                continue
+            got, cands, bst = P.stream_budget(pi, 0, budget, 20000)
+            assert bst.dfs_nodes == min(total_nodes, budget)
+            assert prev <= got <= cnt
+            assert np.array_equal(cands, full[:got])
+            if budget >= total_nodes:
+                assert (got, bst.leaves) == (cnt, st.leaves)
+            prev = got
