@@ -144,7 +144,7 @@ def main():
     ap.add_argument('--node-budget', type=int, default=1 << 22)
     ap.add_argument('--cand-cap', type=int, default=0)
     ap.add_argument('--top-k', type=int, default=32)
-    ap.add_argument('--ab-budget', type=int, default=1 << 24)
+    ap.add_argument('--ab-budget', type=int, default=1 << 20)
     ap.add_argument('--plain-cells', type=int, default=2, help='cells per class timed one per call')
     ap.add_argument('--latency-requests', type=int, default=1500)
     ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')

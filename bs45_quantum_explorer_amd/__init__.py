@@ -20,7 +20,8 @@ HIT, PREREJECT, CLEAN_NO, ABORTED, INTERNAL = 0, 1, 2, 3, 4
 EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy', 'wzfh_last_error',
            'wzfh_num_cells', 'wzfh_cell_info', 'wzfh_live_cells', 'wzfh_order_digest', 'wzfh_stream_cells', 'wzfh_complete',
            'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms',
-           'wzfh_serve_open', 'wzfh_serve_submit', 'wzfh_serve_poll', 'wzfh_serve_close')
+           'wzfh_serve_open', 'wzfh_serve_submit', 'wzfh_serve_poll', 'wzfh_serve_close',
+           'wzfh_join_open', 'wzfh_join_build', 'wzfh_join_probe', 'wzfh_join_close')
 
 
 class WzfhError(RuntimeError):
@@ -97,6 +98,11 @@ def lib():
         L.wzfh_npaf_check.argtypes = [C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int)]
         L.wzfh_flat_score.argtypes = [C.c_int, C.c_uint64, C.c_uint64]
         L.wzfh_flat_score.restype = C.c_longlong
+        L.wzfh_join_open.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.wzfh_join_build.argtypes = [vp, C.c_int, vp]
+        L.wzfh_join_probe.argtypes = [vp, vp]
+        L.wzfh_join_close.argtypes = [vp]
+        L.wzfh_join_close.restype = None
         L.wzfh_last_kernel_ms.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         L.wzfh_serve_open.argtypes = [vp, C.c_int, C.POINTER(ServeOpts), C.POINTER(vp)]
         L.wzfh_serve_submit.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong)]
@@ -252,6 +258,40 @@ class Server:
             rc = lib().wzfh_serve_close(self.h)
             self.h = None
             _check(rc)
+
+
+class JoinResult(C.Structure):
+    _fields_ = [(k, C.c_longlong) for k in ('cd_profiles', 'ab_profiles', 'cd_stream', 'distinct_keys', 'slots',
+                                             'ab_stream', 'hits')] + \
+               [('found', C.c_int), ('reserved', C.c_int)] + [(k, C.c_uint64) for k in 'ABCD']
+
+
+class Join:
+    """The WZ_JOIN22 hash join of one signature (wzfh_join_*): build the C,D table, probe
+    with the A,B stream."""
+
+    def __init__(self, n, sig, **flags):
+        cfg = Config()
+        _check(lib().wzfh_config_default(C.byref(cfg), n, *sig))
+        for k, v in flags.items():
+            setattr(cfg, k, v)
+        h = C.c_void_p()
+        _check(lib().wzfh_join_open(C.byref(cfg), C.byref(h)))
+        self.h, self.n = h, n
+        self.result = JoinResult()
+
+    def build(self, slots_log2=0):
+        _check(lib().wzfh_join_build(self.h, slots_log2, C.byref(self.result)))
+        return self.result
+
+    def probe(self):
+        _check(lib().wzfh_join_probe(self.h, C.byref(self.result)))
+        return self.result
+
+    def close(self):
+        if getattr(self, 'h', None):
+            lib().wzfh_join_close(self.h)
+            self.h = None
 
 
 def flat_score(n, Cw, Dw):

@@ -20,7 +20,7 @@ ARCH = os.environ.get('WZFH_ARCH', 'gfx950')
 
 HIP_FLAGS = ['-O3', '-std=c++17', f'--offload-arch={ARCH}', '-fPIC', '-ffp-contract=off',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wno-unused-result']
-SOURCES = ['wzfh_stream.hip', 'wzfh_complete.hip', 'wzfh_api.hip', 'wzfh_setup.cpp', 'wzfh_serve.cpp']
+SOURCES = ['wzfh_stream.hip', 'wzfh_complete.hip', 'wzfh_api.hip', 'wzfh_join.hip', 'wzfh_setup.cpp', 'wzfh_serve.cpp']
 
 
 def hipcc():

@@ -96,12 +96,12 @@ static int class_count(int L, int c, int m) { return c < L ? (L - c + m - 1) / m
 // Stream-kernel constants of a problem: child tables in DFS digit order (with
 // WZ_FH_STREAM_REV and the C,D root pins folded in), per-depth class info.
 static void build_stream_tables(const wzfh_problem &P, StreamArgs &a) {
-  const int L = P.n, m = P.m, half = L / 2;
+  const int L = P.side_len(), m = P.m, half = L / 2;
   const bool rev = P.cfg.rev != 0, endpos = P.cfg.endpos != 0;
   a.L = L; a.half = half; a.m = m;
   a.odd = L & 1;
   a.cm_mid = half % m;
-  a.nch0 = endpos ? 8 : 16;
+  a.nch0 = (endpos || P.ab_side) ? 8 : 16;
   int tot[8];
   for (int c = 0; c < m; c++) tot[c] = class_count(L, c, m);
   for (int d = 0; d < 32; d++) {
@@ -114,9 +114,12 @@ static void build_stream_tables(const wzfh_problem &P, StreamArgs &a) {
     };
     a.depth_info[d] = (uint32_t)c1 | ((uint32_t)c2 << 8) | ((uint32_t)rem_after(c1) << 16) | ((uint32_t)rem_after(c2) << 24);
   }
+  // root quads: P22_16 on the C,D side (P22_POS with ENDPOS), P22_NEG on the A,B side
+  // (count_pairs22, wz_match.cpp:766-767)
   auto quad_of = [&](int d, int b) -> const int * {
     if (d == 0) {
       const int qi = rev ? a.nch0 - 1 - b : b;
+      if (P.ab_side) return kQuads.neg[qi];
       return endpos ? kQuads.pos[qi] : kQuads.all[qi];
     }
     return kQuads.pos[rev ? 7 - b : b];
@@ -127,7 +130,8 @@ static void build_stream_tables(const wzfh_problem &P, StreamArgs &a) {
     if (b >= a.nch0) { a.quad16[b] = 0; a.root_pin_mask |= 1 << b; continue; }
     const int *v = quad_of(0, b);
     a.quad16[b] = (uint8_t)quad_bits(v);
-    if ((P.pinC && v[0] != 1) || (P.pinD && v[1] != 1)) a.root_pin_mask |= 1 << b;
+    const bool pinX = P.ab_side ? P.cfg.a == 0 : P.pinC, pinY = P.ab_side ? P.cfg.b == 0 : P.pinD;
+    if ((pinX && v[0] != 1) || (pinY && v[1] != 1)) a.root_pin_mask |= 1 << b;
   }
   auto pm_bit = [](int v) { return v > 0 ? 0 : 1; };
   auto sum_bit = [](int s) { return s == 2 ? 0 : (s == 0 ? 1 : 2); };
@@ -308,13 +312,9 @@ void device_free(wzfh_problem &P) {
 
 // ---------------------------------------------------------------------------
 // Stream: launch + exact post-processing.
-struct StreamOut {
-  std::vector<CandRec> cands;                 // sorted by (cell slot, path, ell)
-  std::vector<long long> cell_begin;          // per slot, into cands (ncells+1)
-};
 
 static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncells) {
-  const int half = P.n / 2;
+  const int half = P.side_len() / 2;
   int d0 = o.split_depth;
   if (d0 <= 0) {
     // static items only seed the launch: running items donate their untried subtrees to
@@ -325,13 +325,13 @@ static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncel
   return std::max(1, std::min(d0, std::max(1, half)));
 }
 
-static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o,
+int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o,
                       wzfh_cell_stats *stats, StreamOut &out) {
   DeviceState *S = P.dev;
   WZ_HIP(hipSetDevice(S->device));   // callers may be any host thread (serving workers)
   StreamArgs a = S->base;
   const int d0 = auto_split(P, o, ncells);
-  if (P.n / 2 - d0 > 21) { set_error("split depth too shallow: an item spans at most 21 levels"); return WZFH_EINVAL; }
+  if (P.side_len() / 2 - d0 > 21) { set_error("split depth too shallow: an item spans at most 21 levels"); return WZFH_EINVAL; }
   if (a.nch0 * (double)(1u << std::min(30, 3 * (d0 - 1))) * ncells > 2.0e9) {
     set_error("split depth too deep for this many cells");
     return WZFH_EINVAL;
@@ -344,7 +344,7 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
   // per-cell residual words and root check
   std::vector<uint64_t> res(2 * (size_t)ncells);
   std::vector<int> rootok(ncells);
-  const int L = P.n, m = P.m;
+  const int L = P.side_len(), m = P.m;
   for (int i = 0; i < ncells; i++) {
     const HostCell &c = P.cells[cells[i]];
     uint64_t rx = 0, ry = 0;
@@ -358,7 +358,8 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     res[2 * i] = rx; res[2 * i + 1] = ry; rootok[i] = ok;
   }
   const size_t nstatic = (size_t)ncells * npaths;
-  const size_t ndonate = std::max<size_t>(1u << 22, 64 * (size_t)ncells);
+  // donated records: cleared at every launch, so sized to the batch
+  const size_t ndonate = std::min<size_t>(1u << 24, std::max<size_t>(1u << 18, 16384 * (size_t)ncells));
   const size_t nrec = nstatic + ndonate;
   if (nrec >= 0xFFFFFFF0ull) { set_error("too many work items for one launch"); return WZFH_ECAPACITY; }
   if (S->cell_res.ensure(res.size() * 8) || S->cell_rootok.ensure(ncells * 4) || S->cell_tot.ensure(ncells * 24) ||
@@ -452,7 +453,8 @@ static int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_
     WZ_HIP(hipMemcpyAsync(dqh, S->dq.p, sizeof dqh, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipStreamSynchronize(S->stream));
     if (dqh[5]) {
-      set_error("stream kernel made no progress for 5 s and stopped (internal error)");
+      set_error(dqh[5] == 2 ? "stream kernel met an invalid work record (internal error)"
+                            : "stream kernel made no progress for 5 s and stopped (internal error)");
       return WZFH_EINTERNAL;
     }
     float ms = 0;
@@ -610,6 +612,35 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
   return WZFH_OK;
 }
 
+// A problem handle: setup of one side's cell list and, with a device, its device state.
+// ab_side builds the A,B side of the join instead of the C,D first-hit cells.
+int problem_init(const wzfh_config &cfg_in, bool ab_side, bool with_device, wzfh_problem **out) {
+  *out = nullptr;
+  const wzfh_config *cfg = &cfg_in;
+  const int n = cfg->n;
+  if (n < 4) { set_error("n must be >= 4"); return WZFH_EINVAL; }
+  if (n + 1 > kMaxL - 1) { set_error("n too large for one 64-bit word per sequence"); return WZFH_EINVAL; }
+  if (cfg->a * cfg->a + cfg->b * cfg->b + cfg->c * cfg->c + cfg->d * cfg->d != 4 * n + 2) {
+    set_error("sig norm != 4n+2");
+    return WZFH_EINVAL;
+  }
+  if (cfg->outer_k != 0 && cfg->outer_k != 2 && cfg->outer_k != 3) { set_error("outer_k must be 0, 2 or 3"); return WZFH_EINVAL; }
+  wzfh_problem *P = new wzfh_problem();
+  P->cfg = *cfg;
+  P->ab_side = ab_side;
+  if (P->cfg.abp_cap <= 0) P->cfg.abp_cap = 20000;
+  P->n = n; P->L1 = n + 1; P->m = cfg->m6 ? 6 : 3;
+  if (n >= 36) { P->cfg.thm211b = 1; P->cfg.thm212 = 1; }   // wz_match.cpp:1388
+  P->pinC = cfg->c == 0; P->pinD = cfg->d == 0;
+  if (cfg->canon && (P->pinC || P->pinD)) { P->pinC = P->pinD = false; P->pins_dropped = true; }   // :1661-1662
+  P->ab_prof = cfg->ab_prof < 0 ? (cfg->m6 != 0) : (cfg->ab_prof != 0);
+  int rc = build_problem(*P);
+  if (rc == WZFH_OK && with_device) rc = device_init(*P);
+  if (rc != WZFH_OK) { device_free(*P); delete P; return rc; }
+  *out = P;
+  return WZFH_OK;
+}
+
 }  // namespace wzfh
 
 using namespace wzfh;
@@ -633,28 +664,7 @@ int wzfh_config_default(wzfh_config *cfg, int n, int a, int b, int c, int d) {
 
 static int init_impl(const wzfh_config *cfg, wzfh_problem **out, bool with_device) {
   if (!cfg || !out) return WZFH_EINVAL;
-  *out = nullptr;
-  const int n = cfg->n;
-  if (n < 4) { set_error("n must be >= 4"); return WZFH_EINVAL; }
-  if (n + 1 > kMaxL - 1) { set_error("n too large for one 64-bit word per sequence"); return WZFH_EINVAL; }
-  if (cfg->a * cfg->a + cfg->b * cfg->b + cfg->c * cfg->c + cfg->d * cfg->d != 4 * n + 2) {
-    set_error("sig norm != 4n+2");
-    return WZFH_EINVAL;
-  }
-  if (cfg->outer_k != 0 && cfg->outer_k != 2 && cfg->outer_k != 3) { set_error("outer_k must be 0, 2 or 3"); return WZFH_EINVAL; }
-  wzfh_problem *P = new wzfh_problem();
-  P->cfg = *cfg;
-  if (P->cfg.abp_cap <= 0) P->cfg.abp_cap = 20000;
-  P->n = n; P->L1 = n + 1; P->m = cfg->m6 ? 6 : 3;
-  if (n >= 36) { P->cfg.thm211b = 1; P->cfg.thm212 = 1; }   // wz_match.cpp:1388
-  P->pinC = cfg->c == 0; P->pinD = cfg->d == 0;
-  if (cfg->canon && (P->pinC || P->pinD)) { P->pinC = P->pinD = false; P->pins_dropped = true; }   // :1661-1662
-  P->ab_prof = cfg->ab_prof < 0 ? (cfg->m6 != 0) : (cfg->ab_prof != 0);
-  int rc = build_problem(*P);
-  if (rc == WZFH_OK && with_device) rc = device_init(*P);
-  if (rc != WZFH_OK) { device_free(*P); delete P; return rc; }
-  *out = P;
-  return WZFH_OK;
+  return problem_init(*cfg, false, with_device, out);
 }
 
 int wzfh_init(const wzfh_config *cfg, wzfh_problem **out) { return init_impl(cfg, out, true); }

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/wzfh.h"
+#include "wzfh_kernels.h"
 
 namespace wzfh {
 
@@ -50,6 +51,8 @@ struct wzfh_problem {
   wzfh_config cfg;
   int n = 0, L1 = 0, m = 6;
   bool pinC = false, pinD = false, pins_dropped = false, ab_prof = false;
+  bool ab_side = false;                // the A,B side of the join: streams A,B (length n+1)
+  int side_len() const { return ab_side ? L1 : n; }
   std::vector<wzfh::HostCell> cells;
   std::vector<uint8_t> kept;
   std::vector<int> cell_rows;          // per cell: index into row_off/row_len, -1 = dead
@@ -65,7 +68,16 @@ namespace wzfh {
 // Host setup (wzfh_setup.cpp).
 int build_problem(wzfh_problem &P);
 int side_profiles3(const wzfh_config &c, int n, bool ab_side, std::vector<HostCell> &out);
-// Device side (wzfh_device.hip).
+// The stream of a batch of cells (wzfh_api.hip): per-cell stats and the candidates in
+// each cell's DFS order (CandRec::ell holds the global DFS index G of the leaf).
+struct StreamOut {
+  std::vector<CandRec> cands;                 // by cell slot, then G
+  std::vector<long long> cell_begin;          // per slot, into cands (ncells+1)
+};
+int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o, wzfh_cell_stats *stats,
+               StreamOut &out);
+int problem_init(const wzfh_config &cfg, bool ab_side, bool with_device, wzfh_problem **out);
+// Device side (wzfh_api.hip).
 int device_init(wzfh_problem &P);
 void device_free(wzfh_problem &P);
 }  // namespace wzfh

@@ -58,7 +58,8 @@ struct StreamArgs {
   uint32_t *it_cell, *it_tail;    // donated: cell slot; dead entries after its subtrees
   unsigned long long *dq;         // [0] records allocated, [1] claimed, [2] items in hand,
                                   // [3] capped launches: cells still handing out static items,
-                                  // [4] progress (items finished + donations), [5] stall flag
+                                  // [4] progress (items finished + donations), [5] error flag
+                                  // (1 = no progress, 2 = invalid work record)
   unsigned long long stall_ticks; // watchdog: constant-clock ticks without progress
   uint32_t don_moves, don_low;    // donation period (DFS moves) and pending low-water
   uint32_t don_moves_front;       // donation period of a capped cell's frontier item

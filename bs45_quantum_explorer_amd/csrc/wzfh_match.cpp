@@ -62,6 +62,73 @@ static void print_seq(const char *name, const std::vector<int> &v) {
   printf("};\n");
 }
 
+static void print_solution(int n, const std::vector<int> &A, const std::vector<int> &B, const std::vector<int> &C,
+                           const std::vector<int> &D) {
+  int sa = 0, sb = 0, sc = 0, sd = 0;
+  for (int v : A) sa += v;
+  for (int v : B) sb += v;
+  for (int v : C) sc += v;
+  for (int v : D) sd += v;
+  printf("\n*** BS(%d,%d) FOUND ***\nsig = (%d,%d,%d,%d)\n", n + 1, n, sa, sb, sc, sd);
+  print_seq("A", A); print_seq("B", B); print_seq("C", C); print_seq("D", D);
+  std::vector<int8_t> A8(A.begin(), A.end()), B8(B.begin(), B.end()), C8(C.begin(), C.end()), D8(D.begin(), D.end());
+  int mx = 0;
+  wzfh_npaf_check(n, A8.data(), B8.data(), C8.data(), D8.data(), &mx);
+  printf("VERIFY: max |NPAF[s]| over s=1..%d = %d%s\n", n, mx, mx == 0 ? "  (NPAF==0 confirmed)" : "  (NONZERO!)");
+}
+
+// WZ_JOIN22 (wz_match.cpp:2905-3104): the complete hash join over the Thm 2.2 space, with
+// the reference's output lines and exit codes (0 found, 3 exhausted).
+static int run_join22(int n, int a, int b, int c, int d, Clock::time_point T0) {
+  wzfh_config cfg;
+  wzfh_config_default(&cfg, n, a, b, c, d);
+  if (env_set("WZ_THM211B")) cfg.thm211b = 1;
+  if (env_set("WZ_THM212")) cfg.thm212 = 1;
+  cfg.rev = env_ll("WZ_FH_STREAM_REV", 0) != 0;
+  printf("========================================================\n");
+  printf("  wzfh_match (MI355X) -- HASH-JOIN BS(%d,%d), sig=(%d,%d,%d,%d)\n", n + 1, n, a, b, c, d);
+  printf("========================================================\n");
+  fflush(stdout);
+  wzfh_join *J = nullptr;
+  wzfh_join_result r{};
+  int rc = wzfh_join_open(&cfg, &J);
+  if (rc == WZFH_OK) rc = wzfh_join_build(J, (int)env_ll("WZ_JOIN22_SLOTS_LOG2", 0), &r);
+  if (rc != WZFH_OK) {
+    printf("RESULT: DEVICE ERROR (%s)\n", wzfh_last_error());
+    wzfh_join_close(J);
+    return 2;
+  }
+  int lg = 0;
+  while ((1ll << lg) < r.slots) lg++;
+  const double entry_bytes = 32.0 + 2.0 * ((n + 3) & ~3);
+  const double load = r.slots ? (double)r.distinct_keys / (double)r.slots : 0.0;
+  printf("[profiles] A,B side surviving: %lld   C,D side surviving: %lld\n", r.ab_profiles, r.cd_profiles);
+  printf("[join22v2] key table: 2^%d slots = %g GB\n", lg, (double)r.slots * entry_bytes / 1e9);
+  printf("[join22v2] C,D stream=%g  distinct keys=%lld  (dedup x%g)  table load=%g\n", (double)r.cd_stream,
+         r.distinct_keys, r.distinct_keys ? (double)r.cd_stream / (double)r.distinct_keys : 0.0, load);
+  fflush(stdout);
+  rc = wzfh_join_probe(J, &r);
+  if (rc != WZFH_OK) {
+    printf("RESULT: DEVICE ERROR (%s)\n", wzfh_last_error());
+    wzfh_join_close(J);
+    return 2;
+  }
+  printf("[join22v2] phase 2 done: %lld raw key hits\n", r.hits);
+  const double t = std::chrono::duration<double>(Clock::now() - T0).count();
+  if (r.found) {
+    const std::vector<int> A = unpack(r.A, n + 1), B = unpack(r.B, n + 1), C = unpack(r.C, n), D = unpack(r.D, n);
+    print_solution(n, A, B, C, D);
+    printf("[join22v2] banner flushed AT FIND TIME [%gs] — resolve loop draining, final banner + Time follow\n", t);
+    print_solution(n, A, B, C, D);
+  } else {
+    printf("\n=== JOIN22 EXHAUSTED (n=%d) — no solution in the Thm-2.2 space for sig (%d,%d,%d,%d) ===\n", n, a, b, c, d);
+  }
+  printf("Time: %gs\n", std::chrono::duration<double>(Clock::now() - T0).count());
+  fflush(stdout);
+  wzfh_join_close(J);
+  return r.found ? 0 : 3;
+}
+
 int main(int argc, char **argv) {
   if (argc < 6) {
     fprintf(stderr, "Usage: %s <n> <a> <b> <c> <d>\n  e.g.  %s 10 5 1 4 0   (BS(11,10))\n", argv[0], argv[0]);
@@ -74,11 +141,12 @@ int main(int argc, char **argv) {
     fprintf(stderr, "ERROR: sig norm = %d, expected %d\n", a * a + b * b + c * c + d * d, 4 * n + 2);
     return 1;
   }
+  const auto T0 = Clock::now();
+  if (env_set("WZ_JOIN22")) return run_join22(n, a, b, c, d, T0);
   if (!env_set("WZ_FIRSTHIT")) {
-    fprintf(stderr, "wzfh_match implements the WZ_FIRSTHIT arm only (set WZ_FIRSTHIT=1)\n");
+    fprintf(stderr, "wzfh_match implements the WZ_FIRSTHIT arm and WZ_JOIN22 only (set one of them)\n");
     return 2;
   }
-  const auto T0 = Clock::now();
   wzfh_config cfg;
   wzfh_config_default(&cfg, n, a, b, c, d);
   if (env_set("WZ_THM211B")) cfg.thm211b = 1;

@@ -305,10 +305,32 @@ int side_profiles3(const wzfh_config &c, int n, bool ab_side, std::vector<HostCe
   return WZFH_OK;
 }
 
+// Hall tables: cos / sin of i*j*pi/100 in FP64 from the host libm, exactly as
+// init_hall_tables computes them (wz_match.cpp:106-114).
+static void fill_hall_tables(wzfh_problem &P) {
+  std::memset(P.hcos, 0, sizeof P.hcos);
+  std::memset(P.hsin, 0, sizeof P.hsin);
+  for (int j = 1; j <= 200; j++) {
+    double th = j * M_PI / 100.0;
+    for (int i = 0; i < P.L1 && i < kMaxL; i++) { P.hcos[j][i] = std::cos(i * th); P.hsin[j][i] = std::sin(i * th); }
+  }
+}
+
 int build_problem(wzfh_problem &P) {
   const wzfh_config &c = P.cfg;
   const int n = P.n, L1 = P.L1, m = P.m, tgt = 4 * n + 2;
   const bool t212 = c.thm212 != 0, t211b = c.thm211b != 0;
+
+  if (P.ab_side) {   // the A,B side of the join (WZ_JOIN22): its m = 3 profiles only
+    std::vector<HostCell> ab;
+    int rc = side_profiles3(c, n, true, ab);
+    if (rc != WZFH_OK) return rc;
+    P.cells.swap(ab);
+    P.kept.assign(P.cells.size(), 1);
+    P.cell_rows.assign(P.cells.size(), -1);
+    fill_hall_tables(P);
+    return WZFH_OK;
+  }
 
   NormBits ab3, ab6;
   ab3.build(L1, c.a, c.b, 3, tgt);
@@ -508,14 +530,7 @@ int build_problem(wzfh_problem &P) {
     }
   }
 
-  // Hall tables: cos / sin of i*j*pi/100 in FP64 from the host libm, exactly as
-  // init_hall_tables computes them (wz_match.cpp:106-114).
-  std::memset(P.hcos, 0, sizeof P.hcos);
-  std::memset(P.hsin, 0, sizeof P.hsin);
-  for (int j = 1; j <= 200; j++) {
-    double th = j * M_PI / 100.0;
-    for (int i = 0; i < L1 && i < kMaxL; i++) { P.hcos[j][i] = std::cos(i * th); P.hsin[j][i] = std::sin(i * th); }
-  }
+  fill_hall_tables(P);
   return WZFH_OK;
 }
 

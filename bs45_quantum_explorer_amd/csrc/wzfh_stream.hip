@@ -569,14 +569,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
     if (st == ST_WAIT &&
         __hip_atomic_load(&a.it_state[rec], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == rec_tag(ep, 1)) {
       cs = (int)a.it_cell[rec];
-      if (__hip_atomic_load(&a.cell_crossed[cs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      const uint32_t meta = a.it_meta[rec];
+      base = (int)(meta & 255u);
+      lo_b = (int)((meta >> 8) & 15u);
+      hi_b = (int)((meta >> 12) & 15u);
+      if ((unsigned)cs >= (unsigned)a.ncells || base < 1 || base >= half || lo_b > hi_b || hi_b > 7) {
+        // a record that cannot have been written by a donor: report, touch nothing
+        atomicExch(&a.dq[5], 2ull);
+        cs = 0;
+        st = ST_SKIP;
+      } else if (__hip_atomic_load(&a.cell_crossed[cs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         st = ST_SKIP;
       } else {
         const uint64_t key = a.it_key[rec];
-        const uint32_t meta = a.it_meta[rec];
-        base = (int)(meta & 255u);
-        lo_b = (int)((meta >> 8) & 15u);
-        hi_b = (int)((meta >> 12) & 15u);
         hops = (meta >> 16) & 15u;
         front = false;
         tail = a.it_tail[rec];
@@ -677,7 +682,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
     // its item for a while gives away its untried siblings at the shallowest possible level.
     // Under a cap only the cell's frontier region splits: the item the walk has reached, and
     // a few generations of items donated from it (hops); elsewhere work would be speculative.
-    if ((st == ST_EXPAND || st == ST_BACK) && hungry && since >= (front ? a.don_moves_front : a.don_moves) &&
+    if ((st == ST_EXPAND || st == ST_BACK) && hungry &&
+        since >= ((capped && (front || hops > 0)) ? a.don_moves_front : a.don_moves) &&
         (!capped || front || hops > 0 || a.don_any)) {
       since = 0;
       {
@@ -860,6 +866,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
 __global__ __launch_bounds__(256) void stream_init_kernel(StreamArgs a) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t nstatic = (size_t)a.ncells * (size_t)a.npaths;
+  if (i < (size_t)a.rec_cap) a.it_state[i] = 0;   // no record is published or done yet
   if (i < nstatic) {
     const uint32_t p = (uint32_t)(i % (uint32_t)a.npaths);
     a.it_next[i] = (p + 1 < (uint32_t)a.npaths) ? (uint32_t)i + 1 : kEndItem;
@@ -924,7 +931,7 @@ __global__ __launch_bounds__(256) void stream_keep_kernel(StreamArgs a, unsigned
 
 hipError_t launch_stream(const StreamArgs &a, int blocks, hipStream_t st) {
   const size_t nstatic = (size_t)a.ncells * (size_t)a.npaths;
-  const size_t ninit = std::max(nstatic, (size_t)std::max(a.ncells, 6));
+  const size_t ninit = std::max((size_t)a.rec_cap, (size_t)std::max(a.ncells, 6));
   hipLaunchKernelGGL(stream_init_kernel, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, a);
   hipLaunchKernelGGL(stream_kernel, dim3(blocks), dim3(256), 0, st, a);
   return hipGetLastError();

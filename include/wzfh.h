@@ -193,6 +193,38 @@ int wzfh_serve_submit(wzfh_server *s, int problem, int cell, long long *ticket);
 int wzfh_serve_poll(wzfh_server *s, wzfh_serve_result *out, int max, int min_results, int timeout_ms);
 int wzfh_serve_close(wzfh_server *s);
 
+/* ---- Hash join over the Thm 2.2 space (WZ_JOIN22) ---------------------------- */
+/* Replaces the WZ_JOIN22 mode of main (wz_match.cpp:2905-3104): the C,D stream of every
+ * C,D profile (survive_profiles at m = 3, :439-487, :1521-1522) is stored in an
+ * open-addressed HBM table keyed by its exact autocorrelation vector (-N_C(s) - N_D(s),
+ * s = 1..n; the reference stores a 64-bit FNV of it, :855-871), one entry per distinct
+ * vector (the earliest C,D in stream order); then the A,B stream of every A,B profile
+ * probes it with (N_A(s) + N_B(s)). Equal vectors mean NPAF = 0 at every shift, so every
+ * hit is a solution (no resolve pass, no FNV shadowing). Config: n, a, b, c, d,
+ * thm211b, thm212 and rev are used; m6, canon, order, endpos, ab_prof are not. */
+typedef struct wzfh_join wzfh_join;
+typedef struct {
+  long long cd_profiles, ab_profiles;
+  long long cd_stream;       /* C,D candidates over all C,D profiles                  */
+  long long distinct_keys;   /* table entries (distinct exact vectors)               */
+  long long slots;           /* table slots (a power of two)                         */
+  long long ab_stream;       /* A,B candidates probed                                */
+  long long hits;            /* A,B candidates whose vector is stored: each a solution */
+  int found;                 /* hits > 0                                              */
+  int reserved;
+  uint64_t A, B, C, D;       /* reported solution, as the reference prints it: the
+                                earliest C,D (profile order, DFS order) with a hit and
+                                the last-streamed A,B hitting it                       */
+} wzfh_join_result;
+int wzfh_join_open(const wzfh_config *cfg, wzfh_join **out);
+/* Phase 1 (BUILD, :2929-2968): stream the C,D side and fill the table. slots_log2 = 0
+ * sizes the table to at least twice the stream. Fills cd_*, distinct_keys, slots. */
+int wzfh_join_build(wzfh_join *j, int slots_log2, wzfh_join_result *out);
+/* Phase 2 (STREAM + probe, :2970-3010): stream the A,B side against the table. Fills
+ * ab_*, hits, found and the solution; *out must hold wzfh_join_build's fields. */
+int wzfh_join_probe(wzfh_join *j, wzfh_join_result *out);
+void wzfh_join_close(wzfh_join *j);
+
 /* ---- Exact checks ----------------------------------------------------------- */
 /* Summed NPAF over s = 1..n of int8 +1/-1 sequences (npaf_at, wz_match.cpp:184-192,
  * and the VERIFY line :2640-2646). *max_abs gets max_s |NPAF[s]|. */

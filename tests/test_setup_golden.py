@@ -3,7 +3,7 @@ against the reference's cell lists (tests/golden/stream_cells.json)."""
 import pytest
 
 import bs45_quantum_explorer_amd as wz
-from golden_data import stream_runs
+from golden_data import load, stream_runs
 
 RUNS = stream_runs()
 
@@ -26,3 +26,17 @@ def test_n44_class_list_size_matches_reference_probe():
     assert P.num_cells() == 977601
     px, py, kept, rows = P.cell(0)
     assert sum(px) == 8 and sum(py) == 8
+
+
+def test_join_cd_profiles_match_reference():
+    """survive_profiles on the C,D side (m = 3 with the mod-6 tighten; the JOIN22 and
+    WZ_FH_M6=0 cell list) against the reference's '[profiles] ... C,D side surviving' counts,
+    and every solution the reference's JOIN22 printed has NPAF = 0 at every shift."""
+    for r in load('joins.json'):
+        st = r['settings']
+        P = wz.Problem(r['n'], r['sig'], device=False, m6=0, canon=0, order=0,
+                       thm211b=int(st.get('WZ_THM211B', 0)), thm212=int(st.get('WZ_THM212', 0)))
+        assert P.num_cells() == r['cd_profiles'], (r['n'], r['sig'], st)
+        P.close()
+        if r['found']:
+            assert wz.npaf_max(r['n'], r['A'], r['B'], r['C'], r['D']) == 0
