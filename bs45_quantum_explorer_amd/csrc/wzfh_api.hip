@@ -23,7 +23,12 @@ namespace wzfh {
 hipError_t launch_stream(const StreamArgs &a, int blocks, hipStream_t st);
 hipError_t launch_stream_finalize(const StreamArgs &a, uint64_t *cell_tot, unsigned long long ncands, KeptCand *kept,
                                   unsigned long long *nkept, hipStream_t st);
-hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, hipStream_t st);
+hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, bool lds, size_t lds_bytes,
+                           hipStream_t st);
+size_t complete_lds_bytes(int L, int words);
+hipError_t launch_complete_reduce(const CompleteArgs &a, int *fv, long long *fn, uint64_t *fA, uint64_t *fB,
+                                  hipStream_t st);
+hipError_t complete_set_lds_limit(size_t bytes);
 
 static thread_local std::string g_err;
 std::string set_error(const std::string &msg) { g_err = msg; return msg; }
@@ -57,6 +62,7 @@ struct DBuf {
 
 struct DeviceState {
   int device = 0, num_cu = 0;
+  size_t complete_lds_max = 0;     // dynamic LDS bytes a completer block may use
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // problem constants
@@ -71,7 +77,7 @@ struct DeviceState {
   DBuf it_nodes, it_noff, it_loff, it_coff, it_leaves, it_cands, it_state, it_next, it_key, it_meta, it_cell, it_tail;
   DBuf dq, cell_lock, cell_crossed, cell_wp, cell_pn, cell_pl, cell_pc, cell_open, cell_next, cell_sp;
   uint32_t epoch = 0;              // launch epoch tagging record states
-  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_scratch;
+  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_fv, c_fn, c_fA, c_fB, c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
@@ -252,6 +258,9 @@ int device_init(wzfh_problem &P) {
   hipDeviceProp_t prop;
   WZ_HIP(hipGetDeviceProperties(&prop, S->device));
   S->num_cu = prop.multiProcessorCount;
+  // dynamic LDS of the completer: the device's per-block LDS less the kernel's static tables
+  S->complete_lds_max = prop.sharedMemPerBlock > 8192 ? prop.sharedMemPerBlock - 8192 : 0;
+  if (S->complete_lds_max > 0) WZ_HIP(complete_set_lds_limit(S->complete_lds_max));
   WZ_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
   for (auto &e : S->ev) WZ_HIP(hipEventCreate(&e));
   const size_t tb = 201 * 64 * sizeof(double);
@@ -302,7 +311,7 @@ void device_free(wzfh_problem &P) {
                   &S->it_next, &S->it_key, &S->it_meta, &S->it_cell, &S->it_tail, &S->dq, &S->cell_lock,
                   &S->cell_crossed, &S->cell_wp, &S->cell_pn, &S->cell_pl, &S->cell_pc, &S->cell_open,
                   &S->cell_next, &S->cell_sp, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
-                  &S->c_B, &S->c_scratch})
+                  &S->c_B, &S->c_charges, &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_scratch})
     b->release();
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -438,7 +447,7 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     a.scan_max = std::max(8, std::min(1024, (blocks * 4) / std::max(1, ncells)));
     a.don_moves = 1024;
     a.don_moves_front = 64;
-    a.don_hops = 2;
+    a.don_hops = 1;
     a.don_low = (uint32_t)(blocks * 256 / 4);
     if (const char *e = getenv("WZFH_DONATE_MOVES")) a.don_moves = a.don_moves_front = (uint32_t)std::max(1, atoi(e));
     if (const char *e = getenv("WZFH_DONATE_HOPS")) a.don_hops = (uint32_t)std::min(15, std::max(0, atoi(e)));
@@ -538,6 +547,78 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
 }
 
 // ---------------------------------------------------------------------------
+// One completer launch over ncand candidates (inputs already resolved to row ranges).
+// base > 0 splits every candidate into its 8^base quad paths of depths 0..base-1 (one work
+// item each) and reduces them per candidate; candidates whose budget runs out come back
+// with verdict -1. Adds the kernels' time to *ms.
+static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff, const int *rlen, int ncand,
+                           int maxrows, long long budget, int base, int *v, long long *nd, uint64_t *A,
+                           uint64_t *B, double *ms) {
+  DeviceState *S = P.dev;
+  const int words = std::max(1, (maxrows + 63) / 64);
+  const long long nwork = (long long)ncand << (3 * base);
+  const int blocks = (int)std::max(1ll, std::min((long long)S->num_cu * 8, nwork));
+  const size_t lds_bytes = complete_lds_bytes(P.L1, words);
+  const bool lds = lds_bytes <= S->complete_lds_max;
+  const size_t scratch = lds ? 64 : (size_t)blocks * lds_bytes;
+  const size_t nw = (size_t)nwork;
+  if (S->c_cd.ensure((size_t)ncand * 16) || S->c_roff.ensure(ncand * 4) || S->c_rlen.ensure(ncand * 4) ||
+      S->c_verdict.ensure(nw * 4) || S->c_nodes.ensure(nw * 8) || S->c_A.ensure(nw * 8) || S->c_B.ensure(nw * 8) ||
+      S->c_charges.ensure(nw * 8) || S->c_fv.ensure(ncand * 4) || S->c_fn.ensure(ncand * 8) ||
+      S->c_fA.ensure(ncand * 8) || S->c_fB.ensure(ncand * 8) || S->c_scratch.ensure(scratch) || S->ctr.ensure(64))
+    return WZFH_ENOMEM;
+  WZ_HIP(hipMemcpyAsync(S->c_cd.p, cd, (size_t)ncand * 16, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_roff.p, roff, ncand * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_rlen.p, rlen, ncand * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
+  CompleteArgs a{};
+  a.n = P.n; a.L = P.L1; a.m = P.m;
+  a.absA = std::abs(P.cfg.a); a.absB = std::abs(P.cfg.b);
+  a.outer_k = P.cfg.outer_k; a.canon = !P.cfg.no_canon_ab;
+  a.ncand = ncand; a.budget = budget;
+  a.split_base = base; a.nwork = nwork;
+  a.cd = S->c_cd.as<uint64_t>();
+  a.row_off = S->c_roff.as<int>(); a.row_len = S->c_rlen.as<int>();
+  a.rows = S->rows.p ? S->rows.as<uint64_t>() : S->c_cd.as<uint64_t>();
+  for (int c = 0; c < 8; c++) a.tic[c] = c < P.m ? class_count(P.L1, c, P.m) : 0;
+  a.outer2 = S->outer2.as<uint8_t>(); a.outer3 = S->outer3.as<uint8_t>();
+  a.pos_pack = 0;
+  for (int sb = 0; sb < 16; sb++) a.pos_pack |= (uint64_t)(kOuter.pos_index[sb] + 1) << (4 * sb);
+  a.scratch = S->c_scratch.as<uint64_t>(); a.scratch_words = words;
+  a.work_ctr = S->ctr.as<unsigned long long>();
+  a.verdict = S->c_verdict.as<int>(); a.nodes = S->c_nodes.as<long long>();
+  a.outA = S->c_A.as<uint64_t>(); a.outB = S->c_B.as<uint64_t>();
+  a.charges = S->c_charges.as<long long>();
+  WZ_HIP(hipEventRecord(S->ev[2], S->stream));
+  WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, lds, lds ? lds_bytes : 0, S->stream));
+  int *rv = a.verdict;
+  long long *rn = a.nodes;
+  uint64_t *rA = a.outA, *rB = a.outB;
+  if (base > 0) {
+    rv = S->c_fv.as<int>(); rn = S->c_fn.as<long long>(); rA = S->c_fA.as<uint64_t>(); rB = S->c_fB.as<uint64_t>();
+    WZ_HIP(launch_complete_reduce(a, rv, rn, rA, rB, S->stream));
+  }
+  WZ_HIP(hipEventRecord(S->ev[3], S->stream));
+  WZ_HIP(hipMemcpyAsync(v, rv, ncand * 4, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(nd, rn, ncand * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(A, rA, ncand * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(B, rB, ncand * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipStreamSynchronize(S->stream));
+  float t = 0;
+  WZ_HIP(hipEventElapsedTime(&t, S->ev[2], S->ev[3]));
+  *ms += t;
+  return WZFH_OK;
+}
+
+// Split depth of the completer: the quad paths of depths 0..base-1 become separate work
+// items (8^base per candidate), so one candidate's tree runs on many waves.
+static int complete_split_depth(const wzfh_problem &P) {
+  int base = (P.L1 / 2 >= 6) ? 3 : 0;
+  if (const char *e = getenv("WZFH_COMPLETE_SPLIT")) base = std::max(0, std::min(4, atoi(e)));
+  if (base >= P.L1 / 2) base = 0;
+  return base;
+}
+
 static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
                         wzfh_completion *out) {
   DeviceState *S = P.dev;
@@ -556,45 +637,39 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
     rlen[i] = P.row_len[li];
     maxrows = std::max(maxrows, rlen[i]);
   }
-  const int words = std::max(1, (maxrows + 63) / 64);
-  const int blocks = std::max(1, std::min(S->num_cu * 8, (ncand + 3) / 4));
-  const size_t scratch = (size_t)blocks * 4 * 32 * words * 2 * sizeof(uint64_t);
-  if (S->c_cd.ensure((size_t)ncand * 16) || S->c_roff.ensure(ncand * 4) || S->c_rlen.ensure(ncand * 4) ||
-      S->c_verdict.ensure(ncand * 4) || S->c_nodes.ensure(ncand * 8) || S->c_A.ensure(ncand * 8) ||
-      S->c_B.ensure(ncand * 8) || S->c_scratch.ensure(scratch) || S->ctr.ensure(64))
-    return WZFH_ENOMEM;
-  WZ_HIP(hipMemcpyAsync(S->c_cd.p, cd, (size_t)ncand * 16, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_roff.p, roff.data(), ncand * 4, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_rlen.p, rlen.data(), ncand * 4, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
-  CompleteArgs a{};
-  a.n = P.n; a.L = P.L1; a.m = P.m;
-  a.absA = std::abs(P.cfg.a); a.absB = std::abs(P.cfg.b);
-  a.outer_k = P.cfg.outer_k; a.canon = !P.cfg.no_canon_ab;
-  a.ncand = ncand; a.budget = budget;
-  a.cd = S->c_cd.as<uint64_t>();
-  a.row_off = S->c_roff.as<int>(); a.row_len = S->c_rlen.as<int>();
-  a.rows = S->rows.p ? S->rows.as<uint64_t>() : S->c_cd.as<uint64_t>();
-  for (int c = 0; c < 8; c++) a.tic[c] = c < P.m ? class_count(P.L1, c, P.m) : 0;
-  a.outer2 = S->outer2.as<uint8_t>(); a.outer3 = S->outer3.as<uint8_t>();
-  a.pos_index = S->posidx.as<int8_t>();
-  a.scratch = S->c_scratch.as<uint64_t>(); a.scratch_words = words;
-  a.work_ctr = S->ctr.as<unsigned long long>();
-  a.verdict = S->c_verdict.as<int>(); a.nodes = S->c_nodes.as<long long>();
-  a.outA = S->c_A.as<uint64_t>(); a.outB = S->c_B.as<uint64_t>();
-  WZ_HIP(hipEventRecord(S->ev[2], S->stream));
-  WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, S->stream));
-  WZ_HIP(hipEventRecord(S->ev[3], S->stream));
   std::vector<int> v(ncand);
   std::vector<long long> nd(ncand);
   std::vector<uint64_t> A(ncand), B(ncand);
-  WZ_HIP(hipMemcpyAsync(v.data(), S->c_verdict.p, ncand * 4, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(nd.data(), S->c_nodes.p, ncand * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(A.data(), S->c_A.p, ncand * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(B.data(), S->c_B.p, ncand * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipStreamSynchronize(S->stream));
-  float ms = 0;
-  WZ_HIP(hipEventElapsedTime(&ms, S->ev[2], S->ev[3]));
+  double ms = 0;
+  const int base = complete_split_depth(P);
+  // split launches in chunks (8^base items per candidate), then the candidates whose
+  // budget ran out again as whole trees, for the reference's exact abort counts
+  const int chunk = base > 0 ? std::max(1, (1 << 22) >> (3 * base)) : ncand;
+  for (int c0 = 0; c0 < ncand; c0 += chunk) {
+    const int nc = std::min(chunk, ncand - c0);
+    int rc = complete_launch(P, cd + 2 * (size_t)c0, roff.data() + c0, rlen.data() + c0, nc, maxrows, budget, base,
+                             v.data() + c0, nd.data() + c0, A.data() + c0, B.data() + c0, &ms);
+    if (rc != WZFH_OK) return rc;
+  }
+  if (base > 0) {
+    std::vector<int> re;
+    for (int i = 0; i < ncand; i++) if (v[i] < 0) re.push_back(i);
+    if (!re.empty()) {
+      const int nr = (int)re.size();
+      std::vector<uint64_t> cd2(2 * (size_t)nr);
+      std::vector<int> ro2(nr), rl2(nr), v2(nr);
+      std::vector<long long> nd2(nr);
+      std::vector<uint64_t> A2(nr), B2(nr);
+      for (int j = 0; j < nr; j++) {
+        cd2[2 * j] = cd[2 * (size_t)re[j]]; cd2[2 * j + 1] = cd[2 * (size_t)re[j] + 1];
+        ro2[j] = roff[re[j]]; rl2[j] = rlen[re[j]];
+      }
+      int rc = complete_launch(P, cd2.data(), ro2.data(), rl2.data(), nr, maxrows, budget, 0, v2.data(), nd2.data(),
+                               A2.data(), B2.data(), &ms);
+      if (rc != WZFH_OK) return rc;
+      for (int j = 0; j < nr; j++) { v[re[j]] = v2[j]; nd[re[j]] = nd2[j]; A[re[j]] = A2[j]; B[re[j]] = B2[j]; }
+    }
+  }
   S->last_ms[1] = ms;
   long long units = 0;
   bool internal = false;

@@ -3,10 +3,22 @@
 // Restates fh_complete_ab / fh_ab_search (src/solver/wz_match.cpp:1092-1341) with
 // one candidate per wavefront. The mirror-quad DFS control is wave-uniform (scalar
 // registers); lane s holds the shift-s state: the target T[s] = -CD[s], the partial
-// A,B autocorrelation Dab[s], and reads the remaining-pair bound Kab from a table.
-// A, B are packed 64-bit words (bit i set = element i is -1). A quad placement costs
-// a handful of VALU ops per lane, the bound scan is one compare and a ballot, and the
-// profile-row filter tests 64 rows per instruction against bitsets kept per depth.
+// A,B autocorrelation Dab[s], and the remaining-pair bound Kab[d][s] from an LDS table.
+// A, B are packed 64-bit words (bit i set = element i is -1).
+//
+// A node is expanded in one vector pass: the new pairs that the quad (d, L-1-d) forms
+// with the placed positions reduce to four per-lane partner sums U1, V1, U2, V2 and a
+// cross term, so the Dab change of each of the 8 quads is a signed sum of those, and
+// the bound scan of all 8 children is 8 compares and 8 ballots. The A1 mask, root and
+// reversal canon, early outer check and sum capacity are scalar bit masks over the
+// quads. The quad loop itself then runs on scalar masks: it charges the nodes the
+// reference charges, in its order, and descends into the first child that also passes
+// the profile-row filter (evaluated lazily, in quad order, like the reference).
+//
+// Profile rows: the surviving rows of each depth are bitsets kept in LDS (one word list
+// plus a 64-row bitset per surviving word and depth); row data is read from HBM four
+// words at a time so the loads overlap. When a launch's row lists do not fit in LDS the
+// same code runs on a per-wave global scratch area.
 //
 // Semantics follow the reference exactly, including the order of the per-quad steps
 // (A1 mask, root canon, reversal canon, charge + budget, early outer check, place,
@@ -21,14 +33,7 @@ namespace wzfh {
 
 namespace {
 
-constexpr int kCW = 4;           // waves per block
 constexpr int kLevels = 32;
-
-__constant__ int8_t cNEG[8][4] = {{-1, -1, -1, 1}, {-1, -1, 1, -1}, {-1, 1, -1, -1}, {-1, 1, 1, 1},
-                                  {1, -1, -1, -1}, {1, -1, 1, 1},   {1, 1, -1, 1},   {1, 1, 1, -1}};
-__constant__ int8_t cPOS[8][4] = {{-1, -1, -1, -1}, {-1, -1, 1, 1}, {-1, 1, -1, 1}, {-1, 1, 1, -1},
-                                  {1, -1, -1, 1},   {1, -1, 1, -1}, {1, 1, -1, -1}, {1, 1, 1, 1}};
-__constant__ int8_t cMID[4][2] = {{1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
 
 __device__ __forceinline__ int sgn(uint64_t w, int q) { return 1 - 2 * (int)((w >> q) & 1u); }
 
@@ -42,49 +47,99 @@ __device__ __forceinline__ int put_lane(int val, int at, int old) {
   return ((int)(threadIdx.x & 63) == at) ? val : old;
 }
 
+// Quad k of depth d: (a1, b1, a2, b2) = signs of A[d], B[d], A[L-1-d], B[L-1-d]
+// (P22_NEG / P22_POS of wz_match.cpp): a1 = +1 iff k&4, b1 = +1 iff k&2, a2 = +1 iff k&1,
+// and b2 = a1 b1 a2 at depth >= 1 (quad product +1) or -a1 b1 a2 at depth 0 (product -1).
+// Computed, not looked up: a table indexed by a run-time k is a chain of memory loads.
+__device__ __forceinline__ void quad_signs(int d, int k, int &a1, int &b1, int &a2, int &b2) {
+  a1 = (k & 4) ? 1 : -1;
+  b1 = (k & 2) ? 1 : -1;
+  a2 = (k & 1) ? 1 : -1;
+  b2 = (d == 0 ? -1 : 1) * a1 * b1 * a2;
+}
+
 // Pre-charge filters of quad k at a level: A1 mask, root canon, reversal canon.
 // Returns false when the reference `continue`s before charging the node.
 __device__ __forceinline__ bool precharge_ok(int d, int k, unsigned omask, bool canon, int a_tied, int a_cmp,
                                              int b_tied, int b_cmp) {
   if (d > 0 && !((omask >> k) & 1u)) return false;
-  const int8_t *S = (d == 0) ? cNEG[k] : cPOS[k];
-  if (d == 0) return !canon || (S[0] == 1 && S[1] == 1);
+  int a1, b1, a2, b2;
+  quad_signs(d, k, a1, b1, a2, b2);
+  if (d == 0) return !canon || (a1 == 1 && b1 == 1);
   if (canon) {
-    if (a_tied && S[0] != a_cmp * S[2] && S[0] != 1) return false;
-    if (b_tied && S[1] != b_cmp * S[3] && S[1] != 1) return false;
+    if (a_tied && a1 != a_cmp * a2 && a1 != 1) return false;
+    if (b_tied && b1 != b_cmp * b2 && b1 != 1) return false;
   }
   return true;
 }
 
+// Per-lane partner sums of the quad at depth d (positions i1 = d, i2 = L-1-d) against
+// the placed positions [0, d) and [L-d, L): lane s gets the sums of the A and B signs at
+// the partners of i1 (U1, V1) and of i2 (U2, V2) at distance s, and x = 1 when the two
+// new positions are themselves at distance s (their own pair).
+struct Partners {
+  int U1, V1, U2, V2, x;
+};
+
+__device__ __forceinline__ Partners partners(uint64_t A, uint64_t B, int d, int L, int s, bool valid) {
+  Partners p{0, 0, 0, 0, 0};
+  if (!valid) return p;
+  const int i1 = d, i2 = L - 1 - d;
+  int q = i1 - s;
+  if (q >= 0) { p.U1 += sgn(A, q); p.V1 += sgn(B, q); }
+  q = i1 + s;
+  if (q >= L - d && q < L) { p.U1 += sgn(A, q); p.V1 += sgn(B, q); }
+  q = i2 - s;
+  if (q >= 0 && q < d) { p.U2 += sgn(A, q); p.V2 += sgn(B, q); }
+  if (q == d) p.x = 1;
+  q = i2 + s;
+  if (q >= L - d && q < L) { p.U2 += sgn(A, q); p.V2 += sgn(B, q); }
+  return p;
+}
+
+__device__ __forceinline__ int quad_delta(const Partners &p, int a1, int b1, int a2, int b2) {
+  return a1 * p.U1 + b1 * p.V1 + a2 * p.U2 + b2 * p.V2 + p.x * (a1 * a2 + b1 * b2);
+}
+
 }  // namespace
 
-__global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uint8_t *kab_table) {
-  __shared__ int16_t s_dab[kCW][kLevels][64];
+// One wavefront per block. kLds: the row bitsets live in dynamic LDS; otherwise in the
+// per-wave global scratch area a.scratch.
+template <bool kLds>
+__global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint8_t *kab_table) {
+  __shared__ int16_t s_dab[kLevels][64];
   __shared__ uint8_t s_kab[kLevels][64];
-  for (int i = threadIdx.x; i < kLevels * 64; i += blockDim.x) s_kab[i / 64][i % 64] = kab_table[i];
+  HIP_DYNAMIC_SHARED(uint64_t, dyn_lds)
+  for (int i = threadIdx.x; i < kLevels * 64; i += 64) s_kab[i / 64][i % 64] = kab_table[i];
   __syncthreads();
 
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int n = a.n, L = a.L, m = a.m, half = L / 2;
   const bool canon = a.canon != 0;
   const bool valid = lane >= 1 && lane <= n;
   const int s = lane;
-  const uint64_t gwave = (uint64_t)blockIdx.x * kCW + wv;
-  volatile uint64_t *sbits = a.scratch + gwave * (uint64_t)(kLevels * a.scratch_words * 2);
-  volatile uint64_t *slist = sbits + (uint64_t)kLevels * a.scratch_words;   // u16 lists packed in u64 storage
-  volatile uint16_t *list16 = (volatile uint16_t *)slist;
+  // row bitsets: [half + 1][W] words, then [half + 1][W] u16 word lists
+  const int W = a.scratch_words;
+  const size_t nlw = (size_t)(half + 1) * W;
+  uint64_t *bits = kLds ? dyn_lds : a.scratch + (size_t)blockIdx.x * (nlw + (nlw + 3) / 4);
+  uint16_t *wlist = (uint16_t *)(bits + nlw);
+  const ulonglong2 *rows2 = (const ulonglong2 *)a.rows;
   auto class_field = [](uint64_t w, int c) { return (int)((w >> (8 * c)) & 255u) - 64; };
 
   // DFS states of one level (the iterative form of fh_ab_search's recursion)
   enum { S_ENTER = 0, S_NEXT = 1, S_RETURN = 2, S_FOUND = 3, S_ABORT = 4, S_EXHAUSTED = 5 };
 
+  const int base = a.split_base;            // levels replayed along the item's path
   while (true) {
-    // next candidate: the first active lane takes the ticket, readfirstlane broadcasts it
+    // next work item: the first active lane takes the ticket, readfirstlane broadcasts it
     const unsigned long long act = __ballot(1);
-    int ci_l = 0;
-    if (lane == __ffsll((long long)act) - 1) ci_l = (int)atomicAdd(a.work_ctr, 1ull);
-    const int ci = uni(ci_l);   // the first active lane is the one that took the ticket
-    if (ci >= a.ncand) break;
+    unsigned long long u_l = 0;
+    if (lane == __ffsll((long long)act) - 1) u_l = atomicAdd(a.work_ctr, 1ull);
+    const long long u = (long long)((((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(u_l >> 32))) << 32) |
+                                    (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u_l));
+    if (u >= a.nwork) break;
+    const int ci = (int)(u >> (3 * base));
+    const int path = (int)(u & ((1ll << (3 * base)) - 1));   // digit of depth l at bits 3(base-1-l)
 
     const uint64_t C = a.cd[2 * ci], D = a.cd[2 * ci + 1];
     // T[s] = -CD[s] (wz_match.cpp:1300-1304) from packed words: x_i x_{i+s} = 1 - 2 (b_i ^ b_{i+s}).
@@ -95,7 +150,7 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
       T = -(2 * (n - s) - 2 * dis);
     }
     int verdict = 2;
-    long long total = 0;
+    long long total = 0, cur = 0;   // nodes counted; nodes charged to the budget
     uint64_t A = 0, B = 0;
     // pre-filter (wz_match.cpp:1305-1308): |T[s]| > 2 (L - s) for some s < L
     const bool prerej = any_lane(valid && (T > 2 * (L - s) || -T > 2 * (L - s)));
@@ -104,25 +159,29 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
       const int rlen = uni(a.row_len[ci]);
       const bool rows = roff >= 0;
       const int nwords = rows ? (rlen + 63) / 64 : 0;
-      int lvl_nw = 0;            // lane l: nonzero-word count of S_l
+      int lvl_nw = 0;            // lane l: surviving-word count at depth l
       if (rows) {
         for (int w = lane; w < nwords; w += 64) {
           const int left = rlen - 64 * w;
-          sbits[w] = left >= 64 ? ~0ull : ((1ull << left) - 1);
-          list16[w] = (uint16_t)w;
+          bits[w] = left >= 64 ? ~0ull : ((1ull << left) - 1);
+          wlist[w] = (uint16_t)w;
         }
         lvl_nw = (lane == 0) ? nwords : 0;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
-      // uniform DFS state
-      int d = 0, k = -1, state = S_ENTER;
+      // uniform DFS state of the current node
+      int d = 0, state = S_ENTER;
+      int kcur = -1;             // last quad of this node's loop that was reached
+      unsigned cmask = 0;        // quads passing the pre-charge filters (charged when reached)
+      unsigned vmask = 0;        // ... that also pass the early check, sum capacity and bound
       int sumA = 0, sumB = 0;
       uint64_t PA = 0x4040404040404040ull, PB = PA;   // 8-bit class fields: class sum + 64
-      long long cur = 0;
-      int dab = 0;                // lane s: Dab[s] of the current state
-      int st_k = 0, st_omask = 0, st_ties = 0, st_sums = 0;   // lane l = level l records
-      unsigned omask = 0xFFu;
+      int dab = 0;                // lane s: Dab[s] of the current node
       int a_tied = 1, a_cmp = 0, b_tied = 1, b_cmp = 0;
+      int st_k = 0, st_mask = 0, st_ties = 0, st_sums = 0;   // lane l = level l records
+      Partners pp{0, 0, 0, 0, 0};
+      bool pp_ok = false;         // pp holds the current node's partner sums
 
       while (state < S_FOUND) {
         if (state == S_ENTER) {
@@ -132,7 +191,7 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
             if (L & 1) {
               const int mid = half, cm = mid % m;
               for (int t = 0; t < 4 && !ok_leaf; t++) {
-                const int av = cMID[t][0], bv = cMID[t][1];
+                const int av = (t & 2) ? -1 : 1, bv = (t & 1) ? -1 : 1;   // P22_4 order
                 int delta = 0;
                 if (valid) {
                   if (mid - s >= 0) delta += sgn(A, mid - s) * av + sgn(B, mid - s) * bv;
@@ -142,19 +201,32 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
                 bool ok = (abs(sumA + av) == a.absA) && (abs(sumB + bv) == a.absB);
                 if (ok) ok = !any_lane(valid && (dab + delta != T));
                 if (ok && rows) {
+                  // fh_abp_leaf_ok (wz_match.cpp:1080-1090): a depth-half row equal to the
+                  // class sums; only the middle's class can still differ
                   const int wantk = class_field(PA, cm) + av, wantr = class_field(PB, cm) + bv;
                   const int nw = uni(__builtin_amdgcn_readlane(lvl_nw, half));
+                  const uint16_t *lst = wlist + (size_t)half * W;
+                  const uint64_t *bts = bits + (size_t)half * W;
                   bool hitany = false;
-                  for (int t2 = 0; t2 < nw && !hitany; t2++) {
-                    const int w = uni(list16[half * a.scratch_words * 4 + t2]);
-                    const uint64_t bits = sbits[half * a.scratch_words + w];
-                    bool hit = false;
-                    if ((bits >> lane) & 1) {
-                      const uint64_t kw = a.rows[2 * (roff + 64 * w + lane)], rw = a.rows[2 * (roff + 64 * w + lane) + 1];
-                      const int kk = (int)(int8_t)((kw >> (8 * cm)) & 255), rr = (int)(int8_t)((rw >> (8 * cm)) & 255);
-                      hit = (kk == wantk) && (rr == wantr);
+                  for (int t0 = 0; t0 < nw && !hitany; t0 += 4) {
+                    ulonglong2 r4[4];
+                    bool on[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                      const int w = t0 + u < nw ? uni(lst[t0 + u]) : 0;
+                      on[u] = t0 + u < nw && ((bts[w] >> lane) & 1);
+                      if (on[u]) r4[u] = rows2[roff + 64 * w + lane];
                     }
-                    hitany = any_lane(hit);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                      bool hit = false;
+                      if (on[u]) {
+                        const int kk = (int)(int8_t)((r4[u].x >> (8 * cm)) & 255);
+                        const int rr = (int)(int8_t)((r4[u].y >> (8 * cm)) & 255);
+                        hit = (kk == wantk) && (rr == wantr);
+                      }
+                      hitany = hitany || any_lane(hit);
+                    }
                   }
                   ok = hitany;
                 }
@@ -174,8 +246,11 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
             state = ok_leaf ? S_FOUND : S_RETURN;
             continue;
           }
-          // ---- level entry: A1 outer mask (wz_match.cpp:601-618, 1210-1211)
-          omask = 0xFFu;
+          // ---- level entry. The A1 outer mask (wz_match.cpp:601-618, 1210-1211) is a table
+          // entry: its load is issued first and used last, after the vector bound pass.
+          const int Rl = T - dab;
+          const uint8_t *optr = nullptr;
+          bool ozero = false;
           if (a.outer_k && d > 0) {
             const int kk = a.outer_k;
             if (!(kk < 2 || d < kk || d + kk > L / 2 || (A & 1) || (B & 1))) {
@@ -184,40 +259,152 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
               for (int t = 1; t < kk; t++) {
                 const int sb = ((((A >> t) & 1) ? 0 : 1) << 3) | ((((B >> t) & 1) ? 0 : 1) << 2) |
                                ((((A >> (L - 1 - t)) & 1) ? 0 : 1) << 1) | (((B >> (L - 1 - t)) & 1) ? 0 : 1);
-                const int q = a.pos_index[sb];
+                const int q = (int)((a.pos_pack >> (4 * sb)) & 15u) - 1;
                 if (q < 0) { skip = true; break; }
                 pat += mul * q;
                 mul *= 8;
               }
               if (!skip) {
                 int R[3] = {0, 0, 0};
-                bool zero = false;
                 for (int j = 0; j < kk; j++) {
-                  const int sj = L - 1 - d - j;
-                  R[j] = uni(__builtin_amdgcn_readlane(T - dab, sj));
-                  if ((R[j] & 1) || R[j] < -4 * (j + 1) || R[j] > 4 * (j + 1)) zero = true;
+                  R[j] = uni(__builtin_amdgcn_readlane(Rl, L - 1 - d - j));
+                  if ((R[j] & 1) || R[j] < -4 * (j + 1) || R[j] > 4 * (j + 1)) ozero = true;
                 }
-                if (zero) omask = 0;
-                else if (kk == 2) omask = a.outer2[(pat * 5 + (R[0] / 2 + 2)) * 9 + (R[1] / 2 + 4)];
-                else omask = a.outer3[((pat * 5 + (R[0] / 2 + 2)) * 9 + (R[1] / 2 + 4)) * 13 + (R[2] / 2 + 6)];
+                if (!ozero)
+                  optr = (kk == 2) ? a.outer2 + (pat * 5 + (R[0] / 2 + 2)) * 9 + (R[1] / 2 + 4)
+                                   : a.outer3 + ((pat * 5 + (R[0] / 2 + 2)) * 9 + (R[1] / 2 + 4)) * 13 + (R[2] / 2 + 6);
               }
             }
           }
-          omask = (unsigned)uni((int)omask);
-          if (omask == 0) { state = S_RETURN; continue; }
-          k = -1;
-          st_omask = put_lane((int)omask, d, st_omask);
-          st_ties = put_lane(a_tied | ((a_cmp + 1) << 1) | (b_tied << 3) | ((b_cmp + 1) << 4), d, st_ties);
-          st_sums = put_lane((sumA + 128) | ((sumB + 128) << 8), d, st_sums);
+          const unsigned oraw = optr ? (unsigned)*optr : 0xFFu;
+          // root and reversal canon (wz_match.cpp:1214-1240), early outer check (:1250-1260)
+          // and sum capacity (:1263-1267): scalar per quad
+          unsigned pmask = 0, smask = 0;
+          const int rem = L - 2 * (d + 1);
+          const int Rse = (d > 0) ? uni(__builtin_amdgcn_readlane(Rl, L - 1 - d)) : 0;
+          const int sA0 = sgn(A, 0), sB0 = sgn(B, 0), sAL = sgn(A, L - 1), sBL = sgn(B, L - 1);
+#pragma unroll
+          for (int k = 0; k < 8; k++) {
+            if (!precharge_ok(d, k, 0xFFu, canon, a_tied, a_cmp, b_tied, b_cmp)) continue;
+            pmask |= 1u << k;
+            int a1, b1, a2, b2;
+            quad_signs(d, k, a1, b1, a2, b2);
+            if (d > 0 && sA0 * a2 + sB0 * b2 + sAL * a1 + sBL * b1 != Rse) continue;
+            const int nsA = sumA + a1 + a2, nsB = sumB + b1 + b2;
+            const bool sum_ok = ((abs(a.absA - nsA) <= rem) || (abs(-a.absA - nsA) <= rem)) &&
+                                ((abs(a.absB - nsB) <= rem) || (abs(-a.absB - nsB) <= rem));
+            if (sum_ok) smask |= 1u << k;
+          }
+          // bound scan (:1268-1270) of every remaining quad in one pass: lane s checks
+          // |T[s] - Dab[s] - delta_k[s]| <= Kab[d+1][s]
+          unsigned bmask = 0;
+          if (smask) {
+            pp = partners(A, B, d, L, s, valid);
+            pp_ok = true;
+            const int kb = valid ? (int)s_kab[d + 1][lane] : (1 << 20);
+            const int sg = (d == 0) ? -1 : 1;
+            const int V2s = sg * pp.V2, X2 = (1 + sg) * pp.x;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+              if ((smask >> k) & 1u) {
+                const int a1 = (k & 4) ? 1 : -1, b1 = (k & 2) ? 1 : -1, a2 = (k & 1) ? 1 : -1;
+                const int dk = a1 * pp.U1 + b1 * pp.V1 + a2 * pp.U2 + (a1 * b1 * a2) * V2s + (a1 * a2) * X2;
+                const int e = Rl - dk;
+                if (!any_lane((unsigned)(e + kb) > (unsigned)(2 * kb))) bmask |= 1u << k;
+              }
+            }
+          }
+          // the A1 mask gates the charges: with no quad left the level returns uncharged
+          const unsigned omask = ozero ? 0u : (unsigned)uni((int)oraw);
+          cmask = (omask == 0) ? 0u : (pmask & ((d > 0) ? omask : 0xFFu));
+          vmask = cmask & smask & bmask;
+          kcur = -1;
           state = S_NEXT;
           continue;
         }
         if (state == S_NEXT) {
-          k++;
-          if (k >= 8) { state = S_RETURN; continue; }
-          const int8_t *S = (d == 0) ? cNEG[k] : cPOS[k];
-          const int a1 = S[0], b1 = S[1], a2 = S[2], b2 = S[3];
-          if (!precharge_ok(d, k, omask, canon, a_tied, a_cmp, b_tied, b_cmp)) continue;
+          int k;
+          if (d < base) {
+            // replayed level of a split item: only the path's quad. Its charge belongs to
+            // the first path below it (all later digits 0), so every charge of the levels
+            // above the split is counted by exactly one item, in DFS order.
+            const int sh = 3 * (base - 1 - d);
+            k = (path >> sh) & 7;
+            if (!((cmask >> k) & 1u)) { state = S_EXHAUSTED; continue; }
+            if ((path & ((1 << sh) - 1)) == 0) {
+              total++;
+              if (a.budget > 0 && ++cur > a.budget) { state = S_ABORT; continue; }
+            }
+            if (!((vmask >> k) & 1u)) { state = S_EXHAUSTED; continue; }
+          } else {
+            // the reference's quad loop on masks: every quad of cmask reached is charged;
+            // one of vmask that passes the row filter is descended into
+            const unsigned rest = cmask & (0xFFu << (kcur + 1)) & 0xFFu;
+            if (rest == 0) { state = S_RETURN; continue; }
+            k = __ffs((int)rest) - 1;
+            kcur = k;
+            total++;
+            if (a.budget > 0 && ++cur > a.budget) { state = S_ABORT; continue; }
+            if (!((vmask >> k) & 1u)) continue;
+          }
+          kcur = k;
+          int a1, b1, a2, b2;
+          quad_signs(d, k, a1, b1, a2, b2);
+          const int i1 = d, i2 = L - 1 - d;
+          const int c1 = i1 % m, c2 = i2 % m;
+          const uint64_t nPA = PA + ((uint64_t)(int64_t)a1 << (8 * c1)) + ((uint64_t)(int64_t)a2 << (8 * c2));
+          const uint64_t nPB = PB + ((uint64_t)(int64_t)b1 << (8 * c1)) + ((uint64_t)(int64_t)b2 << (8 * c2));
+          if (rows) {
+            // fh_abp_filter (wz_match.cpp:1055-1078): rows of S_d inside the capacity box of
+            // the two classes this quad changed (the other classes' boxes are unchanged)
+            int r1 = a.tic[c1], r2 = a.tic[c2];
+            for (int i = c1; i < L; i += m) if (i <= d || i >= L - 1 - d) r1--;
+            for (int i = c2; i < L; i += m) if (i <= d || i >= L - 1 - d) r2--;
+            const int p1a = class_field(nPA, c1), p1b = class_field(nPB, c1);
+            const int p2a = class_field(nPA, c2), p2b = class_field(nPB, c2);
+            const int nw = uni(__builtin_amdgcn_readlane(lvl_nw, d));
+            const uint16_t *lst = wlist + (size_t)d * W;
+            const uint64_t *bts = bits + (size_t)d * W;
+            uint16_t *nlst = wlist + (size_t)(d + 1) * W;
+            uint64_t *nbts = bits + (size_t)(d + 1) * W;
+            int cnt = 0;
+            for (int t0 = 0; t0 < nw; t0 += 4) {
+              ulonglong2 r4[4];
+              bool on[4];
+              int w4[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                w4[u] = t0 + u < nw ? uni(lst[t0 + u]) : 0;
+                on[u] = t0 + u < nw && ((bts[w4[u]] >> lane) & 1);
+                if (on[u]) r4[u] = rows2[roff + 64 * w4[u] + lane];
+              }
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                bool ok = false;
+                if (on[u]) {
+                  const uint64_t kw = r4[u].x, rw = r4[u].y;
+                  const int k1 = (int)(int8_t)((kw >> (8 * c1)) & 255), k2 = (int)(int8_t)((kw >> (8 * c2)) & 255);
+                  const int q1 = (int)(int8_t)((rw >> (8 * c1)) & 255), q2 = (int)(int8_t)((rw >> (8 * c2)) & 255);
+                  ok = abs(k1 - p1a) <= r1 && abs(q1 - p1b) <= r1 && abs(k2 - p2a) <= r2 && abs(q2 - p2b) <= r2;
+                }
+                const uint64_t nb = __ballot(ok);
+                if (t0 + u < nw) {
+                  // every lane stores the same uniform values
+                  nbts[w4[u]] = nb;
+                  if (nb) { nlst[cnt] = (uint16_t)w4[u]; cnt++; }
+                }
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            cnt = uni(cnt);
+            lvl_nw = put_lane(cnt, d + 1, lvl_nw);
+            if (cnt == 0) {
+              if (d < base) state = S_EXHAUSTED;
+              continue;
+            }
+          }
+          // ---- descend (the recursion at wz_match.cpp:1281-1287)
           int na_tied = a_tied, na_cmp = a_cmp, nb_tied = b_tied, nb_cmp = b_cmp;
           if (canon) {
             if (d == 0) { na_cmp = a2; nb_cmp = b2; }
@@ -226,99 +413,45 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
               if (b_tied && b1 != b_cmp * b2) nb_tied = 0;
             }
           }
-          total++;
-          if (a.budget > 0 && ++cur > a.budget) { state = S_ABORT; continue; }
-          const int i1 = d, i2 = L - 1 - d;
-          if (d > 0) {   // early outer-correlation check (wz_match.cpp:1256-1259)
-            const int se = L - 1 - d;
-            const int lhs = sgn(A, 0) * a2 + sgn(B, 0) * b2 + sgn(A, L - 1) * a1 + sgn(B, L - 1) * b1;
-            if (lhs != uni(__builtin_amdgcn_readlane(T - dab, se))) continue;
-          }
-          // place i1 then i2 (fh_place): new pairs of each position with placed ones
-          const uint64_t nA = A | ((a1 < 0) ? (1ull << i1) : 0) | ((a2 < 0) ? (1ull << i2) : 0);
-          const uint64_t nB = B | ((b1 < 0) ? (1ull << i1) : 0) | ((b2 < 0) ? (1ull << i2) : 0);
-          int delta = 0;
-          if (valid) {
-            int q = i1 - s;
-            if (q >= 0) delta += sgn(nA, q) * a1 + sgn(nB, q) * b1;
-            q = i1 + s;
-            if (q >= L - d && q < L) delta += sgn(nA, q) * a1 + sgn(nB, q) * b1;
-            q = i2 - s;
-            if (q >= 0 && q <= d) delta += sgn(nA, q) * a2 + sgn(nB, q) * b2;
-            q = i2 + s;
-            if (q >= L - d && q < L) delta += sgn(nA, q) * a2 + sgn(nB, q) * b2;
-          }
-          const int ndab = dab + delta;
-          const int nsA = sumA + a1 + a2, nsB = sumB + b1 + b2, rem = L - 2 * (d + 1);
-          bool prune = !(((abs(a.absA - nsA) <= rem) || (abs(-a.absA - nsA) <= rem)) &&
-                         ((abs(a.absB - nsB) <= rem) || (abs(-a.absB - nsB) <= rem)));
-          if (!prune) {
-            const int kab = s_kab[d + 1][lane];
-            prune = any_lane(valid && abs(T - ndab) > kab);
-          }
-          const int c1 = i1 % m, c2 = i2 % m;
-          const uint64_t nPA = PA + ((uint64_t)(int64_t)a1 << (8 * c1)) + ((uint64_t)(int64_t)a2 << (8 * c2));
-          const uint64_t nPB = PB + ((uint64_t)(int64_t)b1 << (8 * c1)) + ((uint64_t)(int64_t)b2 << (8 * c2));
-          if (!prune && rows) {
-            // fh_abp_filter (wz_match.cpp:1055-1078): rows of S_d inside the capacity box of
-            // the two classes this quad changed (the other classes' boxes are unchanged).
-            int r1 = a.tic[c1], r2 = a.tic[c2];
-            for (int i = c1; i < L; i += m) if (i <= d || i >= L - 1 - d) r1--;
-            for (int i = c2; i < L; i += m) if (i <= d || i >= L - 1 - d) r2--;
-            const int p1a = class_field(nPA, c1), p1b = class_field(nPB, c1);
-            const int p2a = class_field(nPA, c2), p2b = class_field(nPB, c2);
-            const int nw = uni(__builtin_amdgcn_readlane(lvl_nw, d));
-            int cnt = 0;
-            for (int t = 0; t < nw; t++) {
-              const int w = uni(list16[d * a.scratch_words * 4 + t]);
-              const uint64_t bits = sbits[d * a.scratch_words + w];
-              bool ok = false;
-              if ((bits >> lane) & 1) {
-                const uint64_t kw = a.rows[2 * (roff + 64 * w + lane)], rw = a.rows[2 * (roff + 64 * w + lane) + 1];
-                const int k1 = (int)(int8_t)((kw >> (8 * c1)) & 255), k2 = (int)(int8_t)((kw >> (8 * c2)) & 255);
-                const int q1 = (int)(int8_t)((rw >> (8 * c1)) & 255), q2 = (int)(int8_t)((rw >> (8 * c2)) & 255);
-                ok = abs(k1 - p1a) <= r1 && abs(q1 - p1b) <= r1 && abs(k2 - p2a) <= r2 && abs(q2 - p2b) <= r2;
-              }
-              const uint64_t nb = __ballot(ok);
-              if (lane == 0) {
-                sbits[(d + 1) * a.scratch_words + w] = nb;
-                if (nb) list16[(d + 1) * a.scratch_words * 4 + cnt] = (uint16_t)w;
-              }
-              cnt += (nb != 0) ? 1 : 0;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            cnt = uni(cnt);
-            lvl_nw = put_lane(cnt, d + 1, lvl_nw);
-            if (cnt == 0) prune = true;
-          }
-          if (prune) continue;
-          // ---- descend (the recursion at wz_match.cpp:1281-1287)
+          if (!pp_ok) pp = partners(A, B, d, L, s, valid);
+          const int delta = quad_delta(pp, a1, b1, a2, b2);
           st_k = put_lane(k, d, st_k);
-          s_dab[wv][d][lane] = (int16_t)dab;
-          A = nA; B = nB; PA = nPA; PB = nPB;
-          sumA = nsA; sumB = nsB;
-          dab = ndab;
+          st_mask = put_lane((int)(cmask | (vmask << 8)), d, st_mask);
+          st_ties = put_lane(a_tied | ((a_cmp + 1) << 1) | (b_tied << 3) | ((b_cmp + 1) << 4), d, st_ties);
+          st_sums = put_lane((sumA + 128) | ((sumB + 128) << 8), d, st_sums);
+          s_dab[d][lane] = (int16_t)dab;
+          A |= ((a1 < 0) ? (1ull << i1) : 0) | ((a2 < 0) ? (1ull << i2) : 0);
+          B |= ((b1 < 0) ? (1ull << i1) : 0) | ((b2 < 0) ? (1ull << i2) : 0);
+          PA = nPA; PB = nPB;
+          sumA += a1 + a2; sumB += b1 + b2;
+          dab += delta;
           a_tied = na_tied; a_cmp = na_cmp; b_tied = nb_tied; b_cmp = nb_cmp;
           d++;
+          pp_ok = false;
           state = S_ENTER;
           continue;
         }
-        // state == S_RETURN: this level is exhausted; resume the parent's quad loop
-        if (d == 0) { state = S_EXHAUSTED; continue; }
+        // state == S_RETURN: this level is exhausted; resume the parent's quad loop (a split
+        // item ends with its subtree)
+        if (d == base) { state = S_EXHAUSTED; continue; }
         d--;
-        k = uni(__builtin_amdgcn_readlane(st_k, d));
-        omask = (unsigned)uni(__builtin_amdgcn_readlane(st_omask, d));
+        kcur = uni(__builtin_amdgcn_readlane(st_k, d));
+        const int mk = uni(__builtin_amdgcn_readlane(st_mask, d));
+        cmask = (unsigned)mk & 0xFFu;
+        vmask = ((unsigned)mk >> 8) & 0xFFu;
         const int ties = uni(__builtin_amdgcn_readlane(st_ties, d));
         a_tied = ties & 1; a_cmp = ((ties >> 1) & 3) - 1; b_tied = (ties >> 3) & 1; b_cmp = ((ties >> 4) & 3) - 1;
         const int sums = uni(__builtin_amdgcn_readlane(st_sums, d));
         sumA = (sums & 255) - 128; sumB = ((sums >> 8) & 255) - 128;
-        dab = s_dab[wv][d][lane];
-        const int8_t *S = (d == 0) ? cNEG[k] : cPOS[k];
+        dab = s_dab[d][lane];
+        int a1, b1, a2, b2;
+        quad_signs(d, kcur, a1, b1, a2, b2);
         const int i1 = d, i2 = L - 1 - d, c1 = i1 % m, c2 = i2 % m;
         A &= ~((1ull << i1) | (1ull << i2));
         B &= ~((1ull << i1) | (1ull << i2));
-        PA -= ((uint64_t)(int64_t)S[0] << (8 * c1)) + ((uint64_t)(int64_t)S[2] << (8 * c2));
-        PB -= ((uint64_t)(int64_t)S[1] << (8 * c1)) + ((uint64_t)(int64_t)S[3] << (8 * c2));
+        PA -= ((uint64_t)(int64_t)a1 << (8 * c1)) + ((uint64_t)(int64_t)a2 << (8 * c2));
+        PB -= ((uint64_t)(int64_t)b1 << (8 * c1)) + ((uint64_t)(int64_t)b2 << (8 * c2));
+        pp_ok = false;
         state = S_NEXT;
       }
 
@@ -332,14 +465,12 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
         verdict = any_lane(valid && ab != T) ? 4 : 0;
       } else if (state == S_ABORT) {
         // unwinding charges (wz_match.cpp:1245-1247, 1282-1291): each ancestor level whose
-        // loop still has a quad passing the pre-charge filters charges one more node
-        for (int l = d - 1; l >= 0; l--) {
+        // loop still has a quad passing the pre-charge filters charges one more node. A
+        // split item only reports that it ran past the budget (its candidate is rerun whole).
+        for (int l = d - 1; l >= 0 && base == 0; l--) {
           const int kl = uni(__builtin_amdgcn_readlane(st_k, l));
-          const unsigned om = (unsigned)uni(__builtin_amdgcn_readlane(st_omask, l));
-          const int ties = uni(__builtin_amdgcn_readlane(st_ties, l));
-          const int at = ties & 1, ac = ((ties >> 1) & 3) - 1, bt = (ties >> 3) & 1, bc = ((ties >> 4) & 3) - 1;
-          for (int k2 = kl + 1; k2 < 8; k2++)
-            if (precharge_ok(l, k2, om, canon, at, ac, bt, bc)) { total++; break; }
+          const unsigned cm = (unsigned)uni(__builtin_amdgcn_readlane(st_mask, l)) & 0xFFu;
+          if (cm & (0xFFu << (kl + 1)) & 0xFFu) total++;
         }
         verdict = 3;
         A = B = 0;
@@ -351,18 +482,77 @@ __global__ __launch_bounds__(256) void complete_kernel(CompleteArgs a, const uin
       verdict = 1;
     }
     // every lane stores the same uniform values: no lane-0-only block inside the loop
-    a.verdict[ci] = verdict;
-    a.nodes[ci] = total;
-    a.outA[ci] = A;
-    a.outB[ci] = B;
+    a.verdict[u] = verdict;
+    a.nodes[u] = total;
+    a.outA[u] = A;
+    a.outB[u] = B;
+    if (base > 0) a.charges[u] = cur;
   }
 }
 
 }  // namespace wzfh
 
 namespace wzfh {
-hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL(complete_kernel, dim3(blocks), dim3(256), 0, st, a, kab);
+// Per-candidate result of a split launch: the items of a candidate, in path order, are its
+// DFS in order, so running sums of their counted and charged nodes give the reference's
+// counts at every item boundary. The first item with a hit decides, unless the budget ran
+// out before it; a candidate whose charges pass the budget is marked -1 (rerun whole, for
+// the exact node count at the abort).
+__global__ __launch_bounds__(256) void complete_reduce_kernel(CompleteArgs a, int *fv, long long *fn, uint64_t *fA,
+                                                              uint64_t *fB) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= a.ncand) return;
+  const long long np = 1ll << (3 * a.split_base);
+  long long cum_n = 0, cum_c = 0;
+  int v_out = 2;
+  uint64_t A = 0, B = 0;
+  for (long long p = 0; p < np; p++) {
+    const long long u = (long long)ci * np + p;
+    const int v = a.verdict[u];
+    if (v == 1) { v_out = 1; cum_n = 0; break; }
+    if (v == 3) { v_out = -1; break; }
+    const long long c = cum_c + a.charges[u];
+    if (a.budget > 0 && c > a.budget) { v_out = -1; break; }
+    if (v == 0 || v == 4) {
+      v_out = v;
+      cum_n += a.nodes[u];
+      A = a.outA[u];
+      B = a.outB[u];
+      break;
+    }
+    cum_n += a.nodes[u];
+    cum_c = c;
+  }
+  fv[ci] = v_out;
+  fn[ci] = cum_n;
+  fA[ci] = A;
+  fB[ci] = B;
+}
+
+hipError_t launch_complete_reduce(const CompleteArgs &a, int *fv, long long *fn, uint64_t *fA, uint64_t *fB,
+                                  hipStream_t st) {
+  hipLaunchKernelGGL(complete_reduce_kernel, dim3((a.ncand + 255) / 256), dim3(256), 0, st, a, fv, fn, fA, fB);
   return hipGetLastError();
+}
+
+// Bytes of dynamic LDS one wave needs for row lists of `words` 64-row words.
+size_t complete_lds_bytes(int L, int words) {
+  const size_t nlw = (size_t)(L / 2 + 1) * (size_t)words;
+  return nlw * sizeof(uint64_t) + ((nlw * sizeof(uint16_t) + 7) / 8) * 8;
+}
+
+hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, bool lds, size_t lds_bytes,
+                           hipStream_t st) {
+  if (lds) {
+    hipLaunchKernelGGL(complete_kernel<true>, dim3(blocks), dim3(64), lds_bytes, st, a, kab);
+  } else {
+    hipLaunchKernelGGL(complete_kernel<false>, dim3(blocks), dim3(64), 0, st, a, kab);
+  }
+  return hipGetLastError();
+}
+
+hipError_t complete_set_lds_limit(size_t bytes) {
+  return hipFuncSetAttribute((const void *)complete_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
 }
 }  // namespace wzfh

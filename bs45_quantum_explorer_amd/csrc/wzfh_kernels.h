@@ -98,13 +98,19 @@ struct CompleteArgs {
   int n, L, m, absA, absB, outer_k, canon;
   int ncand;
   long long budget;
+  // split launches: work item u is path u % 8^split_base of candidate u / 8^split_base (the
+  // subtree below that quad path); per-item results go to verdict/nodes/outA/outB at u and
+  // the item's budget charges to charges[u]. split_base = 0: one item per candidate.
+  int split_base;
+  long long nwork;
+  long long *charges;
   const uint64_t *cd;          // [ncand][2]
   const int *row_off;          // [ncand] offset into rows, -1 = unconstrained
   const int *row_len;          // [ncand]
   const uint64_t *rows;        // rows packed: k[8], r[8] as int8 (2 words per row)
   int tic[8];                  // class counts of length L at modulus m
   const uint8_t *outer2, *outer3;
-  const int8_t *pos_index;     // [16]
+  uint64_t pos_pack;           // pos_index[sb] + 1 in bits 4sb..4sb+3 (outer-lag pattern digit)
   uint64_t *scratch;           // per-wave row bitsets
   int scratch_words;           // words per level
   unsigned long long *work_ctr;
