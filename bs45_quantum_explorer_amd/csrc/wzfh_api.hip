@@ -77,7 +77,7 @@ struct DeviceState {
   DBuf it_nodes, it_noff, it_loff, it_coff, it_leaves, it_cands, it_state, it_next, it_key, it_meta, it_cell, it_tail;
   DBuf dq, cell_lock, cell_crossed, cell_wp, cell_pn, cell_pl, cell_pc, cell_open, cell_next, cell_sp;
   uint32_t epoch = 0;              // launch epoch tagging record states
-  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_fv, c_fn, c_fA, c_fB, c_scratch;
+  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_fv, c_fn, c_fA, c_fB, c_cc, c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
@@ -311,7 +311,7 @@ void device_free(wzfh_problem &P) {
                   &S->it_next, &S->it_key, &S->it_meta, &S->it_cell, &S->it_tail, &S->dq, &S->cell_lock,
                   &S->cell_crossed, &S->cell_wp, &S->cell_pn, &S->cell_pl, &S->cell_pc, &S->cell_open,
                   &S->cell_next, &S->cell_sp, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
-                  &S->c_B, &S->c_charges, &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_scratch})
+                  &S->c_B, &S->c_charges, &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_cc, &S->c_scratch})
     b->release();
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -448,6 +448,8 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     a.don_moves = 1024;
     a.don_moves_front = 64;
     a.don_hops = 1;
+    a.inner_moves = 16;
+    if (const char *e = getenv("WZFH_INNER_MOVES")) a.inner_moves = std::max(1, std::min(256, atoi(e)));
     a.don_low = (uint32_t)(blocks * 256 / 4);
     if (const char *e = getenv("WZFH_DONATE_MOVES")) a.don_moves = a.don_moves_front = (uint32_t)std::max(1, atoi(e));
     if (const char *e = getenv("WZFH_DONATE_HOPS")) a.don_hops = (uint32_t)std::min(15, std::max(0, atoi(e)));
@@ -565,12 +567,14 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   if (S->c_cd.ensure((size_t)ncand * 16) || S->c_roff.ensure(ncand * 4) || S->c_rlen.ensure(ncand * 4) ||
       S->c_verdict.ensure(nw * 4) || S->c_nodes.ensure(nw * 8) || S->c_A.ensure(nw * 8) || S->c_B.ensure(nw * 8) ||
       S->c_charges.ensure(nw * 8) || S->c_fv.ensure(ncand * 4) || S->c_fn.ensure(ncand * 8) ||
-      S->c_fA.ensure(ncand * 8) || S->c_fB.ensure(ncand * 8) || S->c_scratch.ensure(scratch) || S->ctr.ensure(64))
+      S->c_fA.ensure(ncand * 8) || S->c_fB.ensure(ncand * 8) || S->c_cc.ensure(ncand * 8) ||
+      S->c_scratch.ensure(scratch) || S->ctr.ensure(64))
     return WZFH_ENOMEM;
   WZ_HIP(hipMemcpyAsync(S->c_cd.p, cd, (size_t)ncand * 16, hipMemcpyHostToDevice, S->stream));
   WZ_HIP(hipMemcpyAsync(S->c_roff.p, roff, ncand * 4, hipMemcpyHostToDevice, S->stream));
   WZ_HIP(hipMemcpyAsync(S->c_rlen.p, rlen, ncand * 4, hipMemcpyHostToDevice, S->stream));
   WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
+  WZ_HIP(hipMemsetAsync(S->c_cc.p, 0, (size_t)ncand * 8, S->stream));
   CompleteArgs a{};
   a.n = P.n; a.L = P.L1; a.m = P.m;
   a.absA = std::abs(P.cfg.a); a.absB = std::abs(P.cfg.b);
@@ -580,7 +584,13 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.cd = S->c_cd.as<uint64_t>();
   a.row_off = S->c_roff.as<int>(); a.row_len = S->c_rlen.as<int>();
   a.rows = S->rows.p ? S->rows.as<uint64_t>() : S->c_cd.as<uint64_t>();
-  for (int c = 0; c < 8; c++) a.tic[c] = c < P.m ? class_count(P.L1, c, P.m) : 0;
+  for (int d = 0; d < 32; d++)
+    for (int c = 0; c < 8; c++) {
+      int r = c < P.m ? class_count(P.L1, c, P.m) : 0;
+      for (int i = c; c < P.m && i < P.L1; i += P.m)
+        if (i <= d || i >= P.L1 - 1 - d) r--;
+      a.rcap[d][c] = (uint8_t)std::max(0, r);
+    }
   a.outer2 = S->outer2.as<uint8_t>(); a.outer3 = S->outer3.as<uint8_t>();
   a.pos_pack = 0;
   for (int sb = 0; sb < 16; sb++) a.pos_pack |= (uint64_t)(kOuter.pos_index[sb] + 1) << (4 * sb);
@@ -589,6 +599,7 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.verdict = S->c_verdict.as<int>(); a.nodes = S->c_nodes.as<long long>();
   a.outA = S->c_A.as<uint64_t>(); a.outB = S->c_B.as<uint64_t>();
   a.charges = S->c_charges.as<long long>();
+  a.cand_charged = S->c_cc.as<unsigned long long>();
   WZ_HIP(hipEventRecord(S->ev[2], S->stream));
   WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, lds, lds ? lds_bytes : 0, S->stream));
   int *rv = a.verdict;

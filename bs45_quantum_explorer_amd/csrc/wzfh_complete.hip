@@ -58,20 +58,8 @@ __device__ __forceinline__ void quad_signs(int d, int k, int &a1, int &b1, int &
   b2 = (d == 0 ? -1 : 1) * a1 * b1 * a2;
 }
 
-// Pre-charge filters of quad k at a level: A1 mask, root canon, reversal canon.
-// Returns false when the reference `continue`s before charging the node.
-__device__ __forceinline__ bool precharge_ok(int d, int k, unsigned omask, bool canon, int a_tied, int a_cmp,
-                                             int b_tied, int b_cmp) {
-  if (d > 0 && !((omask >> k) & 1u)) return false;
-  int a1, b1, a2, b2;
-  quad_signs(d, k, a1, b1, a2, b2);
-  if (d == 0) return !canon || (a1 == 1 && b1 == 1);
-  if (canon) {
-    if (a_tied && a1 != a_cmp * a2 && a1 != 1) return false;
-    if (b_tied && b1 != b_cmp * b2 && b1 != 1) return false;
-  }
-  return true;
-}
+// Quad masks (bit k = quad k) of the +1 signs: a1, b1, a2, and b2 at depth >= 1.
+constexpr unsigned kA1Pos = 0xF0u, kB1Pos = 0xCCu, kA2Pos = 0xAAu, kB2Pos = 0x96u;
 
 // Per-lane partner sums of the quad at depth d (positions i1 = d, i2 = L-1-d) against
 // the placed positions [0, d) and [L-d, L): lane s gets the sums of the A and B signs at
@@ -182,6 +170,16 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
       int st_k = 0, st_mask = 0, st_ties = 0, st_sums = 0;   // lane l = level l records
       Partners pp{0, 0, 0, 0, 0};
       bool pp_ok = false;         // pp holds the current node's partner sums
+      // split items of one candidate report their charges 256 at a time; once the sum
+      // passes the budget the candidate is aborted somewhere and is rerun whole, so every
+      // item of it stops
+      auto charged_out = [&](long long c) -> bool {
+        if (base == 0 || a.budget <= 0 || (c & 255) != 0) return false;
+        unsigned long long tot = 0;
+        if (lane == 0) tot = atomicAdd(&a.cand_charged[ci], 256ull) + 256ull;
+        tot = (unsigned long long)__shfl((long long)tot, 0);
+        return tot > (unsigned long long)a.budget;
+      };
 
       while (state < S_FOUND) {
         if (state == S_ENTER) {
@@ -278,23 +276,42 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
           }
           const unsigned oraw = optr ? (unsigned)*optr : 0xFFu;
           // root and reversal canon (wz_match.cpp:1214-1240), early outer check (:1250-1260)
-          // and sum capacity (:1263-1267): scalar per quad
-          unsigned pmask = 0, smask = 0;
-          const int rem = L - 2 * (d + 1);
-          const int Rse = (d > 0) ? uni(__builtin_amdgcn_readlane(Rl, L - 1 - d)) : 0;
-          const int sA0 = sgn(A, 0), sB0 = sgn(B, 0), sAL = sgn(A, L - 1), sBL = sgn(B, L - 1);
-#pragma unroll
-          for (int k = 0; k < 8; k++) {
-            if (!precharge_ok(d, k, 0xFFu, canon, a_tied, a_cmp, b_tied, b_cmp)) continue;
-            pmask |= 1u << k;
-            int a1, b1, a2, b2;
-            quad_signs(d, k, a1, b1, a2, b2);
-            if (d > 0 && sA0 * a2 + sB0 * b2 + sAL * a1 + sBL * b1 != Rse) continue;
-            const int nsA = sumA + a1 + a2, nsB = sumB + b1 + b2;
-            const bool sum_ok = ((abs(a.absA - nsA) <= rem) || (abs(-a.absA - nsA) <= rem)) &&
-                                ((abs(a.absB - nsB) <= rem) || (abs(-a.absB - nsB) <= rem));
-            if (sum_ok) smask |= 1u << k;
+          // and sum capacity (:1263-1267) as masks over the 8 quads (bit k = quad k)
+          unsigned pmask = 0xFFu;
+          if (d == 0) {
+            if (canon) pmask = kA1Pos & kB1Pos;          // root canon: A[0] = B[0] = +1
+          } else if (canon) {
+            // while a prefix ties its reverse, a1 = -1 is dropped when a2 equals the tie sign
+            if (a_tied) pmask &= ~(~kA1Pos & (a_cmp > 0 ? kA2Pos : ~kA2Pos));
+            if (b_tied) pmask &= ~(~kB1Pos & (b_cmp > 0 ? kB2Pos : ~kB2Pos));
           }
+          const unsigned b2pos = (d == 0) ? (~kB2Pos & 0xFFu) : kB2Pos;
+          unsigned emask = 0xFFu;
+          if (d > 0) {
+            // A[0] a2 + B[0] b2 + A[L-1] a1 + B[L-1] b1 must equal the residual of shift L-1-d:
+            // count, per quad, the terms that are +1 (bit-sliced over the 8 quads)
+            const int Rse = uni(__builtin_amdgcn_readlane(Rl, L - 1 - d));
+            const unsigned P1 = (A & 1) ? ~kA2Pos : kA2Pos, P2 = (B & 1) ? ~b2pos : b2pos;
+            const unsigned P3 = ((A >> (L - 1)) & 1) ? ~kA1Pos : kA1Pos, P4 = ((B >> (L - 1)) & 1) ? ~kB1Pos : kB1Pos;
+            const unsigned s1 = P1 ^ P2, c1 = P1 & P2, s2 = P3 ^ P4, c2 = P3 & P4;
+            const unsigned b0 = s1 ^ s2, cr = s1 & s2;
+            const unsigned h1 = c1 ^ c2 ^ cr, h2 = (c1 & c2) | (c1 & cr) | (c2 & cr);
+            const int want = Rse + 4;   // 2 * (number of +1 terms)
+            if ((want & 1) || want < 0 || want > 8) emask = 0;
+            else {
+              const int c = want / 2;
+              emask = ((c & 1) ? b0 : ~b0) & ((c & 2) ? h1 : ~h1) & ((c & 4) ? h2 : ~h2);
+            }
+          }
+          const int rem = L - 2 * (d + 1);
+          auto cap_ok = [&](int absv, int v) { return abs(absv - v) <= rem || abs(-absv - v) <= rem; };
+          const unsigned ap2 = kA1Pos & kA2Pos, am2 = ~kA1Pos & ~kA2Pos & 0xFFu;
+          const unsigned bp2 = kB1Pos & b2pos, bm2 = ~kB1Pos & ~b2pos & 0xFFu;
+          const unsigned amask = (cap_ok(a.absA, sumA + 2) ? ap2 : 0u) | (cap_ok(a.absA, sumA - 2) ? am2 : 0u) |
+                                 (cap_ok(a.absA, sumA) ? (0xFFu & ~(ap2 | am2)) : 0u);
+          const unsigned bmask_s = (cap_ok(a.absB, sumB + 2) ? bp2 : 0u) | (cap_ok(a.absB, sumB - 2) ? bm2 : 0u) |
+                                   (cap_ok(a.absB, sumB) ? (0xFFu & ~(bp2 | bm2)) : 0u);
+          const unsigned smask = pmask & emask & amask & bmask_s & 0xFFu;
           // bound scan (:1268-1270) of every remaining quad in one pass: lane s checks
           // |T[s] - Dab[s] - delta_k[s]| <= Kab[d+1][s]
           unsigned bmask = 0;
@@ -335,6 +352,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
               total++;
               if (a.budget > 0 && ++cur > a.budget) { state = S_ABORT; continue; }
             }
+            if (charged_out(cur)) { state = S_ABORT; continue; }
             if (!((vmask >> k) & 1u)) { state = S_EXHAUSTED; continue; }
           } else {
             // the reference's quad loop on masks: every quad of cmask reached is charged;
@@ -345,6 +363,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
             kcur = k;
             total++;
             if (a.budget > 0 && ++cur > a.budget) { state = S_ABORT; continue; }
+            if (charged_out(cur)) { state = S_ABORT; continue; }
             if (!((vmask >> k) & 1u)) continue;
           }
           kcur = k;
@@ -357,9 +376,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
           if (rows) {
             // fh_abp_filter (wz_match.cpp:1055-1078): rows of S_d inside the capacity box of
             // the two classes this quad changed (the other classes' boxes are unchanged)
-            int r1 = a.tic[c1], r2 = a.tic[c2];
-            for (int i = c1; i < L; i += m) if (i <= d || i >= L - 1 - d) r1--;
-            for (int i = c2; i < L; i += m) if (i <= d || i >= L - 1 - d) r2--;
+            const int r1 = a.rcap[d][c1], r2 = a.rcap[d][c2];
             const int p1a = class_field(nPA, c1), p1b = class_field(nPB, c1);
             const int p2a = class_field(nPA, c2), p2b = class_field(nPB, c2);
             const int nw = uni(__builtin_amdgcn_readlane(lvl_nw, d));

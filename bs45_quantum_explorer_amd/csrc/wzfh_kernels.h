@@ -73,6 +73,7 @@ struct StreamArgs {
   uint32_t *cell_next, *cell_sp;  // next static item to hand out; static items walked past
   uint32_t static_win;
   int scan_max;                   // longest backoff (loop iterations) of idle polling
+  int inner_moves;                // DFS moves a lane makes per pass of the kernel's main loop
   uint32_t *cell_wp;              // first item not yet walked
   unsigned long long *cell_pn, *cell_pl, *cell_pc;   // walked node / leaf / candidate sums
   CandRec *cands;
@@ -104,11 +105,12 @@ struct CompleteArgs {
   int split_base;
   long long nwork;
   long long *charges;
+  unsigned long long *cand_charged;   // split: charges reported by a candidate's items (256 at a time)
   const uint64_t *cd;          // [ncand][2]
   const int *row_off;          // [ncand] offset into rows, -1 = unconstrained
   const int *row_len;          // [ncand]
   const uint64_t *rows;        // rows packed: k[8], r[8] as int8 (2 words per row)
-  int tic[8];                  // class counts of length L at modulus m
+  uint8_t rcap[32][8];         // [d][c]: class-c positions still unplaced after depth d's quad
   const uint8_t *outer2, *outer3;
   uint64_t pos_pack;           // pos_index[sb] + 1 in bits 4sb..4sb+3 (outer-lag pattern digit)
   uint64_t *scratch;           // per-wave row bitsets

@@ -602,44 +602,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
       }
     }
 
-    // ---- 2. one DFS move per working lane. At the item's base level only digits
-    // lo_b..hi_b are its own; a level truncated by a donation ends at the lane's digit.
-    if (st == ST_EXPAND) {
-      const int lo_e = (d == base) ? lo_b : 0, hi_e = (d == base) ? hi_b : 7;
-      const unsigned rmask = (0xFFu >> (7 - hi_e)) & (0xFFu << lo_e);
-      const unsigned mask = child_mask(s_ne, s_eq, s_di[d], RX, RY) & rmask;
-      if (mask == 0) {
-        ell += (uint64_t)(hi_e - lo_e + 1);
-        st = ST_BACK;
-      } else {
-        const int b = __ffs(mask) - 1;
-        ell += (uint64_t)(b - lo_e + 1);
-        apply_quad(s_di[d], d, L, s_q8[b], 1, RX, RY, X, Y);
-        dig |= (uint64_t)b << (3 * (d - base));
-        d++;
-        if (d == half) { has_leaf = true; st = ST_BACK; }
-      }
-    } else if (st == ST_BACK && !has_leaf) {
-      while (true) {
-        if (d == base) { st = ST_DONE; break; }
-        d--;
-        const int sh = 3 * (d - base);
-        const int b = (int)((dig >> sh) & 7);
-        dig &= ~(7ull << sh);
-        apply_quad(s_di[d], d, L, s_q8[b], -1, RX, RY, X, Y);
-        const int hi_e = ((dmask >> d) & 1u) ? b : ((d == base) ? hi_b : 7);
-        const unsigned mask = child_mask(s_ne, s_eq, s_di[d], RX, RY) & (0xFEu << b) & (0xFFu >> (7 - hi_e));
-        if (mask) {
-          const int b2 = __ffs(mask) - 1;
-          ell += (uint64_t)(b2 - b);
-          apply_quad(s_di[d], d, L, s_q8[b2], 1, RX, RY, X, Y);
-          dig |= (uint64_t)b2 << sh;
-          d++;
-          if (d == half) has_leaf = true;
-          else st = ST_EXPAND;
-          break;
+    // ---- 2. DFS moves: a working lane makes up to a.inner_moves moves per pass of this
+    // loop, stopping at a leaf (queued below), at the end of its item or past its node
+    // budget. At the item's base level only digits lo_b..hi_b are its own; a level
+    // truncated by a donation ends at the lane's digit.
+    uint32_t nmoves = 0;
+    {
+      bool go = (st == ST_EXPAND || st == ST_BACK) && !has_leaf;
+      for (int mv = 0; mv < a.inner_moves; mv++) {
+        if (go) {
+          if (st == ST_EXPAND) {
+            const int lo_e = (d == base) ? lo_b : 0, hi_e = (d == base) ? hi_b : 7;
+            const unsigned rmask = (0xFFu >> (7 - hi_e)) & (0xFFu << lo_e);
+            const unsigned mask = child_mask(s_ne, s_eq, s_di[d], RX, RY) & rmask;
+            if (mask == 0) {
+              ell += (uint64_t)(hi_e - lo_e + 1);
+              st = ST_BACK;
+            } else {
+              const int b = __ffs(mask) - 1;
+              ell += (uint64_t)(b - lo_e + 1);
+              apply_quad(s_di[d], d, L, s_q8[b], 1, RX, RY, X, Y);
+              dig |= (uint64_t)b << (3 * (d - base));
+              d++;
+              if (d == half) { has_leaf = true; st = ST_BACK; }
+            }
+          } else {
+            while (true) {
+              if (d == base) { st = ST_DONE; break; }
+              d--;
+              const int sh = 3 * (d - base);
+              const int b = (int)((dig >> sh) & 7);
+              dig &= ~(7ull << sh);
+              apply_quad(s_di[d], d, L, s_q8[b], -1, RX, RY, X, Y);
+              const int hi_e = ((dmask >> d) & 1u) ? b : ((d == base) ? hi_b : 7);
+              const unsigned mask = child_mask(s_ne, s_eq, s_di[d], RX, RY) & (0xFEu << b) & (0xFFu >> (7 - hi_e));
+              if (mask) {
+                const int b2 = __ffs(mask) - 1;
+                ell += (uint64_t)(b2 - b);
+                apply_quad(s_di[d], d, L, s_q8[b2], 1, RX, RY, X, Y);
+                dig |= (uint64_t)b2 << sh;
+                d++;
+                if (d == half) has_leaf = true;
+                else st = ST_EXPAND;
+                break;
+              }
+              ell += (uint64_t)(hi_e - b);
+            }
+          }
+          nmoves++;
+          go = !has_leaf && (st == ST_EXPAND || st == ST_BACK) &&
+               !(a.node_budget > 0 && ell > (uint64_t)a.node_budget);
         }
-        ell += (uint64_t)(hi_e - b);
+        if (!__any(go)) break;
       }
     }
     if (has_leaf) {
@@ -660,8 +674,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
       if (a.cand_cap > 0 && has_leaf &&
           __hip_atomic_load(&a.it_cands[rec], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= (uint32_t)a.cand_cap)
         stop = true;
-      ++since;
-      if (capped && ((since & 127u) == 0 || (a.cand_cap > 0 && has_leaf))) {
+      const uint32_t since0 = since;
+      since += nmoves;
+      if (capped && ((since0 >> 7) != (since >> 7) || (a.cand_cap > 0 && has_leaf))) {
         if (__hip_atomic_load(&a.cell_crossed[cs], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           st = ST_SKIP;
         } else {
