@@ -150,7 +150,7 @@ def main():
     ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')
     ap.add_argument('--max-batch', type=int, default=4096)
     ap.add_argument('--seed', type=int, default=44)
-    ap.add_argument('--cpu-cells', type=int, default=4, help='cells per class in the CPU baseline sample')
+    ap.add_argument('--cpu-cells', type=int, default=8, help='cells per class in the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     a = ap.parse_args()
@@ -286,7 +286,7 @@ def cpu_baseline(a, cls, probs, cells):
     work, oprobs = [], []
     t_setup = time.time()
     for P_gpu, sig, cl in zip(probs, cls, cells):
-        pick = cl[:a.cpu_cells]
+        pick = cl[::max(1, len(cl) // max(1, a.cpu_cells))][:a.cpu_cells]   # spread over the seeded sample
         P = OracleProblem(a.n, sig, canon=1, order=1, cells=[P_gpu.cell(pi)[:2] for pi in pick])
         oprobs.append(P)
         work += [(P, j) for j in range(len(pick))]
@@ -308,7 +308,8 @@ def cpu_baseline(a, cls, probs, cells):
                 log(f'cpu baseline: {k + 1}/{len(work)} cells')
     dt = time.perf_counter() - t0
     return {'value': round(len(work) / dt, 4), 'unit': 'cells/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{len(work)} cells ({a.cpu_cells} per class) of the same workload and unit',
+            'sample': f'{len(work)} cells ({a.cpu_cells} per class, evenly spaced in each class\'s seeded sample) '
+                      f'of the same workload and unit',
             'cands_per_s': round(cands / dt, 1), 'dfs_nodes_per_s': round(nodes / dt, 1), 'seconds': round(dt, 2),
             'setup_s': round(setup, 1), 'cpu_model': _cpu_model()}
 

@@ -81,7 +81,7 @@ struct DeviceState {
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
-  unsigned long long last_telem[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long last_telem[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   double hall_eps = 0.0;           // margin of the matrix-core leaf test
   // the last stream result, reused when the caller asks again for the same cells and
   // options (count first, then fetch) so the kernel runs once
@@ -435,8 +435,8 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     a.angle_rej = S->rej.as<unsigned>();
     a.telem = nullptr;
     if (getenv("WZFH_TELEMETRY") && atoi(getenv("WZFH_TELEMETRY"))) {
-      if (S->telem.ensure(64)) return WZFH_ENOMEM;
-      WZ_HIP(hipMemsetAsync(S->telem.p, 0, 64, S->stream));
+      if (S->telem.ensure(80)) return WZFH_ENOMEM;
+      WZ_HIP(hipMemsetAsync(S->telem.p, 0, 80, S->stream));
       a.telem = S->telem.as<unsigned long long>();
     }
     // persistent grid: as many waves as can be resident (4 per SIMD at the kernel's
@@ -471,7 +471,7 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     float ms = 0;
     WZ_HIP(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
     S->last_ms[0] = ms;
-    if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 64, hipMemcpyDeviceToHost));
+    if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 80, hipMemcpyDeviceToHost));
     if (ctr[1] > cap) { cap = ctr[1] + (ctr[1] >> 2) + 1024; continue; }
     // ---- exact global order on the device; only the cell totals and kept candidates return
     unsigned long long *nkept = S->ctr.as<unsigned long long>() + 2;
@@ -858,15 +858,16 @@ int wzfh_npaf_check(int n, const int8_t *A, const int8_t *B, const int8_t *C, co
 }
 
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units) {
-  if (!p || !p->dev || kind < 0 || kind > 10) return WZFH_EINVAL;
+  if (!p || !p->dev || kind < 0 || kind > 12) return WZFH_EINVAL;
   if (kind == 10) {   // margin of the matrix-core leaf test
     if (ms) *ms = p->dev->hall_eps;
     if (units) *units = 0;
     return WZFH_OK;
   }
   if (kind >= 2) {   // stream-kernel counters of the last launch (WZFH_TELEMETRY=1)
-    if (ms) *ms = (double)p->dev->last_telem[kind - 2];
-    if (units) *units = (long long)p->dev->last_telem[kind - 2];
+    const int slot = kind <= 9 ? kind - 2 : kind - 3;
+    if (ms) *ms = (double)p->dev->last_telem[slot];
+    if (units) *units = (long long)p->dev->last_telem[slot];
     return WZFH_OK;
   }
   if (ms) *ms = p->dev->last_ms[kind];

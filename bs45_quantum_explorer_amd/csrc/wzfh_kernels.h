@@ -91,7 +91,8 @@ struct StreamArgs {
   unsigned long long *telem;   // optional per-launch counters (null = off): [0] cycles in the
                                // leaf test, [1] total wave cycles, [2] flushes, [3] leaves tested,
                                // [4] leaves rechecked in FP64, [5] disagreements (verify mode),
-                               // [6] donations, [7] items donated
+                               // [6] donations, [7] items donated, [8] lane DFS moves,
+                               // [9] wave passes of the main loop
 };
 
 // A,B completer kernel arguments (fh_complete_ab, wz_match.cpp:1092-1341).

@@ -332,11 +332,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
   __shared__ uint8_t s_q8[8], s_q16[16], s_order[200];
   __shared__ unsigned s_rej[200];
   __shared__ Leaf s_q[kWaves][kQueue];
-  __shared__ unsigned long long s_tel[8];   // block telemetry counters (see StreamArgs::telem)
+  __shared__ unsigned long long s_tel[10];   // block telemetry counters (see StreamArgs::telem)
   __shared__ unsigned long long s_wd[kWaves][2];   // watchdog: last progress seen, and when
 
   for (int i = threadIdx.x; i < 256; i += blockDim.x) { s_ne[i] = a.tab_ne[i]; s0_ne[i] = a.tab0_ne[i]; }
-  if (threadIdx.x < 8) s_tel[threadIdx.x] = 0;
+  if (threadIdx.x < 10) s_tel[threadIdx.x] = 0;
   if (threadIdx.x < kWaves) { s_wd[threadIdx.x][0] = ~0ull; s_wd[threadIdx.x][1] = 0; }
   for (int i = threadIdx.x; i < 64; i += blockDim.x) { s_eq[i] = a.tab_eq[i]; s0_eq[i] = a.tab0_eq[i]; }
   for (int i = threadIdx.x; i < 200; i += blockDim.x) { s_order[i] = a.angle_order[i]; s_rej[i] = 0; }
@@ -655,6 +655,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         }
         if (!__any(go)) break;
       }
+      if (telem) {   // lane moves and passes of this wave
+        unsigned long long wm = nmoves;
+        for (int o = 32; o > 0; o >>= 1) wm += __shfl_xor(wm, o);
+        if (lane == 0) { atomicAdd(&s_tel[8], wm); atomicAdd(&s_tel[9], 1ull); }
+      }
     }
     if (has_leaf) {
       lf.X = X;
@@ -873,7 +878,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
   __syncthreads();
   for (int i = threadIdx.x; i < 200; i += blockDim.x)
     if (s_rej[i]) atomicAdd(&a.angle_rej[i], s_rej[i]);
-  if (telem && threadIdx.x < 8) atomicAdd(&a.telem[threadIdx.x], s_tel[threadIdx.x]);
+  if (telem && threadIdx.x < 10) atomicAdd(&a.telem[threadIdx.x], s_tel[threadIdx.x]);
 }
 
 // Per-launch item state: static records start unwalked with no candidates, each linked to

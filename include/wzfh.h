@@ -237,11 +237,13 @@ long long wzfh_flat_score(int n, uint64_t C, uint64_t D);
 /* Average device duration of the named kernel over the last wzfh_search_cells /
  * wzfh_stream_cells / wzfh_complete call, measured with HIP events on the stream the
  * kernel runs on. kind: 0 = stream kernel, 1 = completer kernel (units = DFS nodes).
- * kind 2..7 (with WZFH_TELEMETRY=1 in the environment): the last stream launch's wave
- * cycles in the leaf test, total wave cycles, leaf-test flushes, leaves tested, leaves the
- * matrix-core test left uncertain (decided in FP64), and -- with WZFH_HALL_VERIFY=1, which
- * also runs the FP64 test on every leaf -- leaves whose certain decision disagreed with
- * FP64 (a false reject or false accept; must be 0). kind 8: the leaf test's margin. */
+ * kind 2..9 and 11..12 (with WZFH_TELEMETRY=1 in the environment): the last stream
+ * launch's counters -- 2 wave cycles in the leaf test, 3 total wave cycles, 4 leaf-test
+ * flushes, 5 leaves tested, 6 leaves the matrix-core test left uncertain (decided in FP64),
+ * 7 -- with WZFH_HALL_VERIFY=1, which also runs the FP64 test on every leaf -- leaves whose
+ * certain decision disagreed with FP64 (a false reject or false accept; must be 0),
+ * 8 donations, 9 donated work items, 11 lane DFS moves, 12 wave passes of the main loop.
+ * kind 10: the leaf test's margin. */
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units);
 
 #ifdef __cplusplus

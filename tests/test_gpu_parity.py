@@ -211,3 +211,29 @@ def test_stream_forced_donation_matches_oracle(cap, budget, monkeypatch):
                 for c in cands], (cap, budget, pi)
     finally:
         P.close()
+
+
+@pytest.mark.parametrize('n,sig', [(29, (2, 4, 7, 7)), (44, (1, 7, 8, 8))])
+def test_matrix_core_leaf_test_agrees_with_fp64_on_every_leaf(n, sig, monkeypatch):
+    """The fp16x2 MFMA leaf test with its rigorous margin decides each leaf certain or
+    uncertain; with WZFH_HALL_VERIFY=1 the kernel also runs the reference's FP64 hall_ok
+    on every leaf and counts certain decisions that disagree with it (false rejects and
+    false accepts). The count must be 0, and the stream must equal the normal-mode stream."""
+    P = wz.Problem(n, sig, canon=1, order=1)
+    try:
+        cells = P.live_cells()[:: max(1, len(P.live_cells()) // 24)][:24]
+        budget = 1 << 18
+        monkeypatch.setenv('WZFH_TELEMETRY', '1')
+        st0, offs0, cd0 = P.stream(cells, node_budget=budget)
+        leaves0 = P.last_kernel(5)[1]
+        monkeypatch.setenv('WZFH_HALL_VERIFY', '1')
+        st1, offs1, cd1 = P.stream(cells, node_budget=budget)
+        tested, disagree = P.last_kernel(5)[1], P.last_kernel(7)[1]
+        assert leaves0 > 0 and tested > 0
+        assert disagree == 0, (n, disagree, tested)
+        assert [(s.cand, s.leaves, s.hall_ok, s.dfs_nodes) for s in st0] == \
+               [(s.cand, s.leaves, s.hall_ok, s.dfs_nodes) for s in st1]
+        assert np.array_equal(offs0, offs1) and np.array_equal(cd0, cd1)
+        assert P.last_kernel(10)[0] > 0   # the margin in use
+    finally:
+        P.close()
