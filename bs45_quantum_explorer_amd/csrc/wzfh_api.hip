@@ -26,8 +26,6 @@ hipError_t launch_stream_finalize(const StreamArgs &a, uint64_t *cell_tot, unsig
 hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, bool lds, size_t lds_bytes,
                            hipStream_t st);
 size_t complete_lds_bytes(int L, int words);
-hipError_t launch_complete_reduce(const CompleteArgs &a, int *fv, long long *fn, uint64_t *fA, uint64_t *fB,
-                                  hipStream_t st);
 hipError_t complete_set_lds_limit(size_t bytes);
 
 static thread_local std::string g_err;
@@ -77,7 +75,8 @@ struct DeviceState {
   DBuf it_nodes, it_noff, it_loff, it_coff, it_leaves, it_cands, it_state, it_next, it_key, it_meta, it_cell, it_tail;
   DBuf dq, cell_lock, cell_crossed, cell_wp, cell_pn, cell_pl, cell_pc, cell_open, cell_next, cell_sp;
   uint32_t epoch = 0;              // launch epoch tagging record states
-  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_fv, c_fn, c_fA, c_fB, c_cc, c_scratch;
+  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_icharged, c_ebase, c_epath, c_ebud,
+       c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
@@ -311,7 +310,8 @@ void device_free(wzfh_problem &P) {
                   &S->it_next, &S->it_key, &S->it_meta, &S->it_cell, &S->it_tail, &S->dq, &S->cell_lock,
                   &S->cell_crossed, &S->cell_wp, &S->cell_pn, &S->cell_pl, &S->cell_pc, &S->cell_open,
                   &S->cell_next, &S->cell_sp, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
-                  &S->c_B, &S->c_charges, &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_cc, &S->c_scratch})
+                  &S->c_B, &S->c_charges, &S->c_icharged, &S->c_ebase, &S->c_epath, &S->c_ebud,
+                  &S->c_scratch})
     b->release();
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -549,38 +549,63 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
 }
 
 // ---------------------------------------------------------------------------
-// One completer launch over ncand candidates (inputs already resolved to row ranges).
-// base > 0 splits every candidate into its 8^base quad paths of depths 0..base-1 (one work
-// item each) and reduces them per candidate; candidates whose budget runs out come back
-// with verdict -1. Adds the kernels' time to *ms.
-static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff, const int *rlen, int ncand,
-                           int maxrows, long long budget, int base, int *v, long long *nd, uint64_t *A,
-                           uint64_t *B, double *ms) {
+// One completer launch over nent entries (candidates, or subtrees of candidates below a
+// quad path; inputs already resolved to row ranges). ext > 0 gives every entry 8^ext work
+// items (its quad paths ext levels further down) and reduces them per entry: verdict 0/1/2/4
+// is final, -1 marks an entry whose budget runs out inside item j (charges cc and counted
+// nodes nd before it, its own charges cj). ext = 0 runs every entry as one item whose budget
+// abort is counted exactly. Adds the kernels' time to *ms.
+struct CompleteEntry {
+  int cand;              // candidate index of the caller's arrays
+  long long path;        // quad path above the entry's subtree (3 bits per depth)
+  int base;              // depth of the subtree root
+  long long budget;      // charges left to the candidate's budget at the subtree start
+  long long pre_nodes;   // counted nodes of the candidate before the subtree
+};
+
+static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff, const int *rlen,
+                           const std::vector<CompleteEntry> &ents, int maxrows, long long budget, int ext, int *v,
+                           long long *nd, uint64_t *A, uint64_t *B, int *fj, long long *fcc, long long *fcur,
+                           double *ms) {
   DeviceState *S = P.dev;
+  const int nent = (int)ents.size();
   const int words = std::max(1, (maxrows + 63) / 64);
-  const long long nwork = (long long)ncand << (3 * base);
+  const long long nwork = (long long)nent << (3 * ext);
   const int blocks = (int)std::max(1ll, std::min((long long)S->num_cu * 8, nwork));
   const size_t lds_bytes = complete_lds_bytes(P.L1, words);
   const bool lds = lds_bytes <= S->complete_lds_max;
   const size_t scratch = lds ? 64 : (size_t)blocks * lds_bytes;
   const size_t nw = (size_t)nwork;
-  if (S->c_cd.ensure((size_t)ncand * 16) || S->c_roff.ensure(ncand * 4) || S->c_rlen.ensure(ncand * 4) ||
+  std::vector<uint64_t> ecd(2 * (size_t)nent);
+  std::vector<int> ero(nent), erl(nent), ebase(nent);
+  std::vector<long long> epath(nent), ebud(nent);
+  for (int e = 0; e < nent; e++) {
+    const CompleteEntry &x = ents[e];
+    ecd[2 * e] = cd[2 * (size_t)x.cand]; ecd[2 * e + 1] = cd[2 * (size_t)x.cand + 1];
+    ero[e] = roff[x.cand]; erl[e] = rlen[x.cand];
+    ebase[e] = x.base; epath[e] = x.path; ebud[e] = x.budget;
+  }
+  if (S->c_cd.ensure((size_t)nent * 16) || S->c_roff.ensure(nent * 4) || S->c_rlen.ensure(nent * 4) ||
+      S->c_ebase.ensure(nent * 4) || S->c_epath.ensure(nent * 8) || S->c_ebud.ensure(nent * 8) ||
       S->c_verdict.ensure(nw * 4) || S->c_nodes.ensure(nw * 8) || S->c_A.ensure(nw * 8) || S->c_B.ensure(nw * 8) ||
-      S->c_charges.ensure(nw * 8) || S->c_fv.ensure(ncand * 4) || S->c_fn.ensure(ncand * 8) ||
-      S->c_fA.ensure(ncand * 8) || S->c_fB.ensure(ncand * 8) || S->c_cc.ensure(ncand * 8) ||
+      S->c_charges.ensure(nw * 8) || S->c_icharged.ensure(nw * 8) ||
       S->c_scratch.ensure(scratch) || S->ctr.ensure(64))
     return WZFH_ENOMEM;
-  WZ_HIP(hipMemcpyAsync(S->c_cd.p, cd, (size_t)ncand * 16, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_roff.p, roff, ncand * 4, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_rlen.p, rlen, ncand * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_cd.p, ecd.data(), (size_t)nent * 16, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_roff.p, ero.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_rlen.p, erl.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_ebase.p, ebase.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_epath.p, epath.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(S->c_ebud.p, ebud.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
   WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
-  WZ_HIP(hipMemsetAsync(S->c_cc.p, 0, (size_t)ncand * 8, S->stream));
+  if (ext > 0) WZ_HIP(hipMemsetAsync(S->c_icharged.p, 0, nw * 8, S->stream));
   CompleteArgs a{};
   a.n = P.n; a.L = P.L1; a.m = P.m;
   a.absA = std::abs(P.cfg.a); a.absB = std::abs(P.cfg.b);
   a.outer_k = P.cfg.outer_k; a.canon = !P.cfg.no_canon_ab;
-  a.ncand = ncand; a.budget = budget;
-  a.split_base = base; a.nwork = nwork;
+  a.ncand = nent; a.budget = budget;
+  a.split_ext = ext; a.nwork = nwork;
+  a.ent_path = S->c_epath.as<long long>(); a.ent_base = S->c_ebase.as<int>(); a.ent_budget = S->c_ebud.as<long long>();
   a.cd = S->c_cd.as<uint64_t>();
   a.row_off = S->c_roff.as<int>(); a.row_len = S->c_rlen.as<int>();
   a.rows = S->rows.p ? S->rows.as<uint64_t>() : S->c_cd.as<uint64_t>();
@@ -599,35 +624,66 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.verdict = S->c_verdict.as<int>(); a.nodes = S->c_nodes.as<long long>();
   a.outA = S->c_A.as<uint64_t>(); a.outB = S->c_B.as<uint64_t>();
   a.charges = S->c_charges.as<long long>();
-  a.cand_charged = S->c_cc.as<unsigned long long>();
+  a.item_charged = S->c_icharged.as<long long>();
   WZ_HIP(hipEventRecord(S->ev[2], S->stream));
   WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, lds, lds ? lds_bytes : 0, S->stream));
-  int *rv = a.verdict;
-  long long *rn = a.nodes;
-  uint64_t *rA = a.outA, *rB = a.outB;
-  if (base > 0) {
-    rv = S->c_fv.as<int>(); rn = S->c_fn.as<long long>(); rA = S->c_fA.as<uint64_t>(); rB = S->c_fB.as<uint64_t>();
-    WZ_HIP(launch_complete_reduce(a, rv, rn, rA, rB, S->stream));
-  }
   WZ_HIP(hipEventRecord(S->ev[3], S->stream));
-  WZ_HIP(hipMemcpyAsync(v, rv, ncand * 4, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(nd, rn, ncand * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(A, rA, ncand * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(B, rB, ncand * 8, hipMemcpyDeviceToHost, S->stream));
+  std::vector<int> iv(nw);
+  std::vector<long long> in(nw), ic(nw);
+  std::vector<uint64_t> iA(nw), iB(nw);
+  WZ_HIP(hipMemcpyAsync(iv.data(), a.verdict, nw * 4, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(in.data(), a.nodes, nw * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(ic.data(), a.charges, nw * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(iA.data(), a.outA, nw * 8, hipMemcpyDeviceToHost, S->stream));
+  WZ_HIP(hipMemcpyAsync(iB.data(), a.outB, nw * 8, hipMemcpyDeviceToHost, S->stream));
   WZ_HIP(hipStreamSynchronize(S->stream));
+  // Per-entry result: the items of an entry, in path order, are its DFS in order, so
+  // running sums of their counted and charged nodes give the reference's counts at every
+  // item boundary. The first item with a hit decides unless the budget ran out before it;
+  // otherwise the first item whose charges pass the budget holds the abort point (-1, with
+  // that item, the charges and counted nodes before it and its own charges).
+  const long long np = 1ll << (3 * ext);
+  for (int e = 0; e < nent; e++) {
+    long long cum_n = 0, cum_c = 0, cj = 0;
+    int vo = 2, jo = -1;
+    uint64_t Ao = 0, Bo = 0;
+    for (long long p = 0; p < np; p++) {
+      const size_t u = (size_t)((long long)e * np + p);
+      if (iv[u] == 1) { vo = 1; cum_n = 0; break; }
+      if (ext > 0 && (iv[u] == 3 || (budget > 0 && cum_c + ic[u] > ents[e].budget))) {
+        vo = -1; jo = (int)p; cj = ic[u];
+        break;
+      }
+      cum_n += in[u];
+      if (iv[u] == 0 || iv[u] == 3 || iv[u] == 4) {   // a hit, or (unsplit) an exactly counted abort
+        vo = iv[u]; Ao = iA[u]; Bo = iB[u];
+        break;
+      }
+      cum_c += ic[u];
+    }
+    v[e] = vo; nd[e] = cum_n; A[e] = Ao; B[e] = Bo;
+    fj[e] = jo; fcc[e] = cum_c; fcur[e] = cj;
+  }
   float t = 0;
   WZ_HIP(hipEventElapsedTime(&t, S->ev[2], S->ev[3]));
   *ms += t;
   return WZFH_OK;
 }
 
-// Split depth of the completer: the quad paths of depths 0..base-1 become separate work
-// items (8^base per candidate), so one candidate's tree runs on many waves.
-static int complete_split_depth(const wzfh_problem &P) {
-  int base = (P.L1 / 2 >= 6) ? 3 : 0;
-  if (const char *e = getenv("WZFH_COMPLETE_SPLIT")) base = std::max(0, std::min(4, atoi(e)));
-  if (base >= P.L1 / 2) base = 0;
-  return base;
+// Split of the completer: every candidate's quad paths of depths 0..2 become separate work
+// items (8^3 per candidate) so one candidate's tree runs on many waves; the subtree that holds
+// a budget abort is split the same way again (3 more levels) until it is small, then walked as
+// one item that counts the abort exactly.
+static int complete_split_ext(const wzfh_problem &P) {
+  int ext = (P.L1 / 2 >= 6) ? 3 : 0;
+  if (const char *e = getenv("WZFH_COMPLETE_SPLIT")) ext = std::max(0, std::min(4, atoi(e)));
+  if (ext >= P.L1 / 2) ext = 0;
+  return ext;
+}
+// an abort item this small (charges) is walked as one item; WZFH_COMPLETE_EXACT_ITEM overrides
+static long long exact_item_charges() {
+  const char *e = getenv("WZFH_COMPLETE_EXACT_ITEM");
+  return e ? std::max(0ll, atoll(e)) : 16384;
 }
 
 static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
@@ -648,50 +704,63 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
     rlen[i] = P.row_len[li];
     maxrows = std::max(maxrows, rlen[i]);
   }
-  std::vector<int> v(ncand);
-  std::vector<long long> nd(ncand);
-  std::vector<uint64_t> A(ncand), B(ncand);
+  std::vector<int> fv(ncand, 2);
+  std::vector<long long> fnd(ncand, 0);
+  std::vector<uint64_t> fA(ncand, 0), fB(ncand, 0);
   double ms = 0;
-  const int base = complete_split_depth(P);
-  // split launches in chunks (8^base items per candidate), then the candidates whose
-  // budget ran out again as whole trees, for the reference's exact abort counts
-  const int chunk = base > 0 ? std::max(1, (1 << 22) >> (3 * base)) : ncand;
-  for (int c0 = 0; c0 < ncand; c0 += chunk) {
-    const int nc = std::min(chunk, ncand - c0);
-    int rc = complete_launch(P, cd + 2 * (size_t)c0, roff.data() + c0, rlen.data() + c0, nc, maxrows, budget, base,
-                             v.data() + c0, nd.data() + c0, A.data() + c0, B.data() + c0, &ms);
-    if (rc != WZFH_OK) return rc;
-  }
-  if (base > 0) {
-    std::vector<int> re;
-    for (int i = 0; i < ncand; i++) if (v[i] < 0) re.push_back(i);
-    if (!re.empty()) {
-      const int nr = (int)re.size();
-      std::vector<uint64_t> cd2(2 * (size_t)nr);
-      std::vector<int> ro2(nr), rl2(nr), v2(nr);
-      std::vector<long long> nd2(nr);
-      std::vector<uint64_t> A2(nr), B2(nr);
-      for (int j = 0; j < nr; j++) {
-        cd2[2 * j] = cd[2 * (size_t)re[j]]; cd2[2 * j + 1] = cd[2 * (size_t)re[j] + 1];
-        ro2[j] = roff[re[j]]; rl2[j] = rlen[re[j]];
+  const int ext0 = complete_split_ext(P);
+  const int max_base = std::min(P.L1 / 2 - 1, 21);   // paths are 3 bits per depth in 64 bits
+  const long long exact_item = exact_item_charges();
+  std::vector<CompleteEntry> pending(ncand);
+  for (int i = 0; i < ncand; i++) pending[i] = CompleteEntry{i, 0, 0, budget, 0};
+  // entries to split further, and entries walked as one item (exact abort count)
+  std::vector<CompleteEntry> split = ext0 > 0 ? pending : std::vector<CompleteEntry>{};
+  std::vector<CompleteEntry> whole = ext0 > 0 ? std::vector<CompleteEntry>{} : pending;
+  while (!split.empty() || !whole.empty()) {
+    std::vector<CompleteEntry> next_split, next_whole;
+    for (int pass = 0; pass < 2; pass++) {
+      std::vector<CompleteEntry> &ents = pass == 0 ? split : whole;
+      const int ext = pass == 0 ? ext0 : 0;
+      const int chunk = std::max(1, (1 << 22) >> (3 * ext));
+      for (size_t c0 = 0; c0 < ents.size(); c0 += chunk) {
+        const std::vector<CompleteEntry> part(ents.begin() + c0, ents.begin() + std::min(ents.size(), c0 + chunk));
+        const int ne = (int)part.size();
+        std::vector<int> v(ne), j(ne);
+        std::vector<long long> nd(ne), cc(ne), cj(ne);
+        std::vector<uint64_t> A(ne), B(ne);
+        int rc = complete_launch(P, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, v.data(), nd.data(),
+                                 A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms);
+        if (rc != WZFH_OK) return rc;
+        for (int e = 0; e < ne; e++) {
+          const CompleteEntry &x = part[e];
+          if (v[e] >= 0) {   // decided: hit, clean, pre-rejected, or an exactly counted abort
+            fv[x.cand] = v[e];
+            fnd[x.cand] = x.pre_nodes + nd[e];
+            fA[x.cand] = A[e];
+            fB[x.cand] = B[e];
+            continue;
+          }
+          // the budget runs out inside item j: its subtree with the charges that remain
+          CompleteEntry y{x.cand, (x.path << (3 * ext)) | j[e], x.base + ext, x.budget - cc[e], x.pre_nodes + nd[e]};
+          if (cj[e] <= exact_item || y.base + ext0 > max_base) next_whole.push_back(y);
+          else next_split.push_back(y);
+        }
       }
-      int rc = complete_launch(P, cd2.data(), ro2.data(), rl2.data(), nr, maxrows, budget, 0, v2.data(), nd2.data(),
-                               A2.data(), B2.data(), &ms);
-      if (rc != WZFH_OK) return rc;
-      for (int j = 0; j < nr; j++) { v[re[j]] = v2[j]; nd[re[j]] = nd2[j]; A[re[j]] = A2[j]; B[re[j]] = B2[j]; }
     }
+    split.swap(next_split);
+    whole.swap(next_whole);
   }
   S->last_ms[1] = ms;
   long long units = 0;
   bool internal = false;
   for (int i = 0; i < ncand; i++) {
-    out[i].verdict = v[i];
+    out[i].verdict = fv[i];
     out[i].reserved = 0;
-    out[i].nodes = nd[i];
-    out[i].A = A[i];
-    out[i].B = B[i];
-    units += nd[i];
-    if (v[i] == 4) internal = true;
+    out[i].nodes = fnd[i];
+    out[i].A = fA[i];
+    out[i].B = fB[i];
+    units += fnd[i];
+    if (fv[i] == 4) internal = true;
   }
   S->last_units[1] = units;
   if (internal) { set_error("completer FOUND failed the exact NPAF recheck"); return WZFH_EINTERNAL; }

@@ -41,6 +41,11 @@ __device__ __forceinline__ int sgn(uint64_t w, int q) { return 1 - 2 * (int)((w 
 // of the DFS is a scalar branch and no loop exit is ever lowered to an exec mask.
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ bool any_lane(bool p) { return uni(__any(p) ? 1 : 0) != 0; }
+__device__ __forceinline__ long long uni64(long long v) {
+  const unsigned lo = (unsigned)uni((int)(unsigned)(unsigned long long)v);
+  const unsigned hi = (unsigned)uni((int)(unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
 
 // Writes a wave-uniform value into lane `at` of a per-lane record (lane = DFS level).
 __device__ __forceinline__ int put_lane(int val, int at, int old) {
@@ -117,7 +122,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
   // DFS states of one level (the iterative form of fh_ab_search's recursion)
   enum { S_ENTER = 0, S_NEXT = 1, S_RETURN = 2, S_FOUND = 3, S_ABORT = 4, S_EXHAUSTED = 5 };
 
-  const int base = a.split_base;            // levels replayed along the item's path
+  const int ext = a.split_ext;              // path digits this launch adds per entry
   while (true) {
     // next work item: the first active lane takes the ticket, readfirstlane broadcasts it
     const unsigned long long act = __ballot(1);
@@ -126,8 +131,13 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
     const long long u = (long long)((((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(u_l >> 32))) << 32) |
                                     (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u_l));
     if (u >= a.nwork) break;
-    const int ci = (int)(u >> (3 * base));
-    const int path = (int)(u & ((1ll << (3 * base)) - 1));   // digit of depth l at bits 3(base-1-l)
+    const int ci = (int)(u >> (3 * ext));                     // entry
+    const int sub = (int)(u & ((1ll << (3 * ext)) - 1));
+    const int base = uni((a.ent_base ? a.ent_base[ci] : 0) + ext);   // levels replayed along the path
+    const long long path = uni64(((a.ent_path ? a.ent_path[ci] : 0ll) << (3 * ext)) | sub);   // depth l: bits 3(base-1-l)
+    // a budgeted launch (a.budget > 0) gives each entry the charges left to it, possibly 0
+    const bool limited = a.budget > 0;
+    const long long bud = uni64(a.ent_budget ? a.ent_budget[ci] : a.budget);
 
     const uint64_t C = a.cd[2 * ci], D = a.cd[2 * ci + 1];
     // T[s] = -CD[s] (wz_match.cpp:1300-1304) from packed words: x_i x_{i+s} = 1 - 2 (b_i ^ b_{i+s}).
@@ -170,17 +180,19 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
       int st_k = 0, st_mask = 0, st_ties = 0, st_sums = 0;   // lane l = level l records
       Partners pp{0, 0, 0, 0, 0};
       bool pp_ok = false;         // pp holds the current node's partner sums
-      // split items of one candidate report their charges 256 at a time; once the sum
-      // passes the budget the candidate is aborted somewhere and is rerun whole, so every
-      // item of it stops
+      // split items report their charges 256 at a time; an item whose entry's earlier items
+      // have together charged more than the budget lies wholly past the abort point and stops
+      // (its result is never used). An item before or at the abort point never stops early.
       auto charged_out = [&](long long c) -> bool {
-        if (base == 0 || a.budget <= 0 || (c & 255) != 0) return false;
-        unsigned long long tot = 0;
-        if (lane == 0) tot = atomicAdd(&a.cand_charged[ci], 256ull) + 256ull;
-        tot = (unsigned long long)__shfl((long long)tot, 0);
-        return tot > (unsigned long long)a.budget;
+        if (ext == 0 || !limited || (c & 255) != 0) return false;
+        if (lane == 0) __hip_atomic_store(&a.item_charged[u], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((c & 4095) != 0 || sub == 0) return false;
+        long long sum = 0;
+        for (int q = lane; q < sub; q += 64)
+          sum += __hip_atomic_load(&a.item_charged[u - sub + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        return uni((int)(sum > bud)) != 0;
       };
-
       while (state < S_FOUND) {
         if (state == S_ENTER) {
           if (d == half) {
@@ -346,11 +358,11 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
             // the first path below it (all later digits 0), so every charge of the levels
             // above the split is counted by exactly one item, in DFS order.
             const int sh = 3 * (base - 1 - d);
-            k = (path >> sh) & 7;
+            k = (int)((path >> sh) & 7);
             if (!((cmask >> k) & 1u)) { state = S_EXHAUSTED; continue; }
-            if ((path & ((1 << sh) - 1)) == 0) {
+            if ((path & ((1ll << sh) - 1)) == 0) {
               total++;
-              if (a.budget > 0 && ++cur > a.budget) { state = S_ABORT; continue; }
+              if (limited && ++cur > bud) { state = S_ABORT; continue; }
             }
             if (charged_out(cur)) { state = S_ABORT; continue; }
             if (!((vmask >> k) & 1u)) { state = S_EXHAUSTED; continue; }
@@ -362,7 +374,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
             k = __ffs((int)rest) - 1;
             kcur = k;
             total++;
-            if (a.budget > 0 && ++cur > a.budget) { state = S_ABORT; continue; }
+            if (limited && ++cur > bud) { state = S_ABORT; continue; }
             if (charged_out(cur)) { state = S_ABORT; continue; }
             if (!((vmask >> k) & 1u)) continue;
           }
@@ -482,9 +494,10 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
         verdict = any_lane(valid && ab != T) ? 4 : 0;
       } else if (state == S_ABORT) {
         // unwinding charges (wz_match.cpp:1245-1247, 1282-1291): each ancestor level whose
-        // loop still has a quad passing the pre-charge filters charges one more node. A
-        // split item only reports that it ran past the budget (its candidate is rerun whole).
-        for (int l = d - 1; l >= 0 && base == 0; l--) {
+        // loop still has a quad passing the pre-charge filters charges one more node (the
+        // path's replayed levels included). In a split launch the item only reports that it
+        // stopped: the abort point is then located and rerun as one item.
+        for (int l = d - 1; l >= 0 && ext == 0; l--) {
           const int kl = uni(__builtin_amdgcn_readlane(st_k, l));
           const unsigned cm = (unsigned)uni(__builtin_amdgcn_readlane(st_mask, l)) & 0xFFu;
           if (cm & (0xFFu << (kl + 1)) & 0xFFu) total++;
@@ -503,55 +516,13 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
     a.nodes[u] = total;
     a.outA[u] = A;
     a.outB[u] = B;
-    if (base > 0) a.charges[u] = cur;
+    a.charges[u] = cur;
   }
 }
 
 }  // namespace wzfh
 
 namespace wzfh {
-// Per-candidate result of a split launch: the items of a candidate, in path order, are its
-// DFS in order, so running sums of their counted and charged nodes give the reference's
-// counts at every item boundary. The first item with a hit decides, unless the budget ran
-// out before it; a candidate whose charges pass the budget is marked -1 (rerun whole, for
-// the exact node count at the abort).
-__global__ __launch_bounds__(256) void complete_reduce_kernel(CompleteArgs a, int *fv, long long *fn, uint64_t *fA,
-                                                              uint64_t *fB) {
-  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ci >= a.ncand) return;
-  const long long np = 1ll << (3 * a.split_base);
-  long long cum_n = 0, cum_c = 0;
-  int v_out = 2;
-  uint64_t A = 0, B = 0;
-  for (long long p = 0; p < np; p++) {
-    const long long u = (long long)ci * np + p;
-    const int v = a.verdict[u];
-    if (v == 1) { v_out = 1; cum_n = 0; break; }
-    if (v == 3) { v_out = -1; break; }
-    const long long c = cum_c + a.charges[u];
-    if (a.budget > 0 && c > a.budget) { v_out = -1; break; }
-    if (v == 0 || v == 4) {
-      v_out = v;
-      cum_n += a.nodes[u];
-      A = a.outA[u];
-      B = a.outB[u];
-      break;
-    }
-    cum_n += a.nodes[u];
-    cum_c = c;
-  }
-  fv[ci] = v_out;
-  fn[ci] = cum_n;
-  fA[ci] = A;
-  fB[ci] = B;
-}
-
-hipError_t launch_complete_reduce(const CompleteArgs &a, int *fv, long long *fn, uint64_t *fA, uint64_t *fB,
-                                  hipStream_t st) {
-  hipLaunchKernelGGL(complete_reduce_kernel, dim3((a.ncand + 255) / 256), dim3(256), 0, st, a, fv, fn, fA, fB);
-  return hipGetLastError();
-}
-
 // Bytes of dynamic LDS one wave needs for row lists of `words` 64-row words.
 size_t complete_lds_bytes(int L, int words) {
   const size_t nlw = (size_t)(L / 2 + 1) * (size_t)words;

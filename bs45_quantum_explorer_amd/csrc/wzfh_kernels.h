@@ -100,13 +100,18 @@ struct CompleteArgs {
   int n, L, m, absA, absB, outer_k, canon;
   int ncand;
   long long budget;
-  // split launches: work item u is path u % 8^split_base of candidate u / 8^split_base (the
-  // subtree below that quad path); per-item results go to verdict/nodes/outA/outB at u and
-  // the item's budget charges to charges[u]. split_base = 0: one item per candidate.
-  int split_base;
+  // Work items. Entry e (a candidate, or a subtree of one below a quad path) has 8^split_ext
+  // items: item u = e * 8^split_ext + x walks the subtree below path (ent_path[e] << 3 ext) | x
+  // of depth ent_base[e] + split_ext (null arrays: path 0, depth 0), with the entry's budget
+  // ent_budget[e] (null: budget). Per-item results go to verdict/nodes/outA/outB/charges at u.
+  // Inputs cd/row_off/row_len are indexed by entry.
+  int split_ext;
   long long nwork;
+  const long long *ent_path;
+  const int *ent_base;
+  const long long *ent_budget;
   long long *charges;
-  unsigned long long *cand_charged;   // split: charges reported by a candidate's items (256 at a time)
+  long long *item_charged;        // split: charges each item has reported so far (256 at a time)
   const uint64_t *cd;          // [ncand][2]
   const int *row_off;          // [ncand] offset into rows, -1 = unconstrained
   const int *row_len;          // [ncand]

@@ -123,6 +123,20 @@ def test_complete_only_matches_reference(key):
     _complete_group(GROUPS[key])
 
 
+BUDGETED = sorted(k for k in GROUPS if any(c['budget'] > 0 for c in GROUPS[k]))
+
+
+@pytest.mark.parametrize('split', [0, 2, 3])
+@pytest.mark.parametrize('key', BUDGETED, ids=lambda k: f'n{k[0]}-{k[1]}-k{k[2]}')
+def test_complete_split_abort_search_matches_reference(key, split, monkeypatch):
+    """Budgeted completions with the candidate split into 8^split quad-path items, the item
+    holding a budget abort split again down to single items (threshold 1 charge), and the
+    last one walked whole for the exact abort count (split 0: every candidate unsplit)."""
+    monkeypatch.setenv('WZFH_COMPLETE_SPLIT', str(split))
+    monkeypatch.setenv('WZFH_COMPLETE_EXACT_ITEM', '1')
+    _complete_group([c for c in GROUPS[key] if c['budget'] > 0])
+
+
 @pytest.mark.parametrize('n,sig,budgets', [(13, (2, 4, 3, 5), (0, 30)), (19, (0, 2, 5, 7), (0, 500)),
                                            (12, (1, 7, 0, 0), (0, 7))])
 def test_in_cell_completion_with_profile_rows_matches_oracle(n, sig, budgets):
