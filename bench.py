@@ -153,6 +153,8 @@ def main():
     ap.add_argument('--cpu-cells', type=int, default=8, help='cells per class in the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
+    ap.add_argument('--tune-cells', type=int, default=64, help='cells of the autotuning sample')
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; cells shard by cell index mod G across GPUs '
@@ -168,6 +170,17 @@ def main():
         cells.append(sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000 + i))
         log(f'setup class {i + 1}/{len(cls)} {sig}: {P.num_cells()} cells')
     setup_s = time.time() - t_setup
+    # launch settings (speed only): tuned once on a separate sample of the first class and
+    # applied to every class (the tuner checks that each setting returns identical results)
+    tune = None
+    if not a.no_tune:
+        t0 = time.time()
+        sample = sample_cells(probs[0].live_cells(), a.tune_cells, a.seed * 7919 + 1)
+        t = probs[0].autotune(sample, a.top_k, a.ab_budget, cand_cap=a.cand_cap, node_budget=a.node_budget)
+        for P in probs[1:]:
+            P.set_tuning(t)
+        tune = dict(t.as_dict(), seconds=round(time.time() - t0, 1), sample_cells=len(sample))
+        log(f'autotune: {tune}')
     reqs = [(pi, c) for pi, cl in enumerate(cells) for c in cl]
     total_cells = len(reqs)
     log(f'setup: {len(cls)} classes, {total_cells} cells, {setup_s:.1f} s')
@@ -267,7 +280,7 @@ def main():
         'plain_latency_ms': {'p50': pct(plain_lat, 50), 'p99': pct(plain_lat, 99), 'cells': nplain},
         'hits': hits, 'dfs_nodes_per_step': dfs_nodes, 'ab_nodes_per_step': ab_nodes,
         'plain_vs_served_mismatch': mismatch, 'setup_s': round(setup_s, 1),
-        'roofline': roofline, 'cpu_baseline': cpu,
+        'roofline': roofline, 'cpu_baseline': cpu, 'autotune': tune,
     }
     print(json.dumps(line), flush=True)
     for P in probs:

@@ -21,7 +21,8 @@ EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy',
            'wzfh_num_cells', 'wzfh_cell_info', 'wzfh_live_cells', 'wzfh_order_digest', 'wzfh_stream_cells', 'wzfh_complete',
            'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms',
            'wzfh_serve_open', 'wzfh_serve_submit', 'wzfh_serve_poll', 'wzfh_serve_close',
-           'wzfh_join_open', 'wzfh_join_build', 'wzfh_join_probe', 'wzfh_join_close')
+           'wzfh_join_open', 'wzfh_join_build', 'wzfh_join_probe', 'wzfh_join_close',
+           'wzfh_autotune', 'wzfh_get_tuning', 'wzfh_set_tuning')
 
 
 class WzfhError(RuntimeError):
@@ -59,6 +60,16 @@ class SearchResult(C.Structure):
     _fields_ = [('stream', CellStats), ('completed', C.c_int), ('hit', C.c_int), ('hit_rank', C.c_longlong),
                 ('hit_idx', C.c_longlong), ('ab_nodes', C.c_longlong), ('clean_no', C.c_int), ('aborted', C.c_int),
                 ('A', C.c_uint64), ('B', C.c_uint64), ('C', C.c_uint64), ('D', C.c_uint64)]
+
+
+class Tuning(C.Structure):
+    """wzfh_tuning: launch settings that change speed only."""
+    _fields_ = [('inner_moves', C.c_int), ('donate_hops', C.c_int), ('static_window', C.c_int),
+                ('complete_split', C.c_int), ('idle_sleep', C.c_int), ('trials', C.c_int),
+                ('stream_ms', C.c_double), ('complete_ms', C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ServeOpts(C.Structure):
@@ -108,6 +119,9 @@ def lib():
         L.wzfh_serve_submit.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong)]
         L.wzfh_serve_poll.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
         L.wzfh_serve_close.argtypes = [vp]
+        L.wzfh_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(SearchOpts), C.POINTER(Tuning)]
+        L.wzfh_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.wzfh_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
     return _lib
 
@@ -221,6 +235,23 @@ class Problem:
         if rc not in (OK, EINTERNAL):
             _check(rc)
         return [res[i] for i in range(nc)]
+
+    def autotune(self, cells, top_k, budget, cand_cap=0, node_budget=0):
+        """Times the launch settings on these cells with this search unit and keeps the fastest
+        (wzfh_autotune); raises if any setting changes a result."""
+        cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
+        opts = SearchOpts(StreamOpts(cand_cap, node_budget, 0, 0), top_k, 1, budget)
+        t = Tuning()
+        _check(lib().wzfh_autotune(self.h, cells.ctypes.data, len(cells), C.byref(opts), C.byref(t)))
+        return t
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().wzfh_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, t):
+        _check(lib().wzfh_set_tuning(self.h, C.byref(t)))
 
     def last_kernel(self, kind):
         ms, units = C.c_double(), C.c_longlong()

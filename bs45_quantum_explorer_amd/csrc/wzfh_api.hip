@@ -61,6 +61,7 @@ struct DBuf {
 struct DeviceState {
   int device = 0, num_cu = 0;
   size_t complete_lds_max = 0;     // dynamic LDS bytes a completer block may use
+  wzfh_tuning tune{16, 1, 4, 3, 64, 0, 0.0, 0.0};   // launch settings (speed only)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // problem constants
@@ -426,7 +427,7 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     a.cell_open = S->cell_open.as<int>();
     a.cell_next = S->cell_next.as<uint32_t>();
     a.cell_sp = S->cell_sp.as<uint32_t>();
-    a.static_win = 4;
+    a.static_win = (uint32_t)S->tune.static_window;
     a.stall_ticks = 500000000ull;   // 5 s of the 100 MHz constant clock
 
     if (const char *e = getenv("WZFH_STATIC_WINDOW")) a.static_win = (uint32_t)std::max(1, atoi(e));
@@ -447,8 +448,10 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     a.scan_max = std::max(8, std::min(1024, (blocks * 4) / std::max(1, ncells)));
     a.don_moves = 1024;
     a.don_moves_front = 64;
-    a.don_hops = 1;
-    a.inner_moves = 16;
+    a.don_hops = (uint32_t)S->tune.donate_hops;
+    a.inner_moves = S->tune.inner_moves;
+    a.idle_sleep = S->tune.idle_sleep;
+    if (const char *e = getenv("WZFH_IDLE_SLEEP")) a.idle_sleep = std::max(0, std::min(256, atoi(e)));
     if (const char *e = getenv("WZFH_INNER_MOVES")) a.inner_moves = std::max(1, std::min(256, atoi(e)));
     a.don_low = (uint32_t)(blocks * 256 / 4);
     if (const char *e = getenv("WZFH_DONATE_MOVES")) a.don_moves = a.don_moves_front = (uint32_t)std::max(1, atoi(e));
@@ -675,7 +678,7 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
 // a budget abort is split the same way again (3 more levels) until it is small, then walked as
 // one item that counts the abort exactly.
 static int complete_split_ext(const wzfh_problem &P) {
-  int ext = (P.L1 / 2 >= 6) ? 3 : 0;
+  int ext = (P.L1 / 2 >= 6) ? P.dev->tune.complete_split : 0;
   if (const char *e = getenv("WZFH_COMPLETE_SPLIT")) ext = std::max(0, std::min(4, atoi(e)));
   if (ext >= P.L1 / 2) ext = 0;
   return ext;
@@ -1007,6 +1010,105 @@ int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_
   }
   p->dev->last_ms[0] = sms; p->dev->last_ms[1] = cms;
   p->dev->last_units[0] = su; p->dev->last_units[1] = cu;
+  return WZFH_OK;
+}
+
+
+int wzfh_get_tuning(const wzfh_problem *p, wzfh_tuning *out) {
+  if (!p || !out) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  *out = p->dev->tune;
+  return WZFH_OK;
+}
+
+int wzfh_set_tuning(wzfh_problem *p, const wzfh_tuning *t) {
+  if (!p || !t) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  if (t->inner_moves < 1 || t->inner_moves > 256 || t->donate_hops < 0 || t->donate_hops > 15 ||
+      t->static_window < 1 || t->complete_split < 0 || t->complete_split > 4 || t->idle_sleep < 0 ||
+      t->idle_sleep > 256) {
+    set_error("tuning value out of range");
+    return WZFH_EINVAL;
+  }
+  p->dev->tune = *t;
+  return WZFH_OK;
+}
+
+// Coordinate descent over the launch settings on a sample of cells. Every trial must give
+// the same per-cell results as the first: the settings only move work between lanes.
+int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts, wzfh_tuning *out) {
+  if (!p || !cells || ncells <= 0 || !opts) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  for (int i = 0; i < ncells; i++)
+    if (cells[i] < 0 || cells[i] >= (int)p->cells.size()) { set_error("cell index out of range"); return WZFH_EINVAL; }
+  DeviceState *S = p->dev;
+  wzfh_tuning best = S->tune;
+  int trials = 0;
+  std::vector<wzfh_cell_stats> ref_st;
+  auto stream_trial = [&](const wzfh_tuning &t, double *ms) -> int {
+    S->tune = t;
+    std::vector<wzfh_cell_stats> st(ncells);
+    StreamOut so;
+    int rc = run_stream(*p, cells, ncells, opts->stream, st.data(), so);
+    if (rc != WZFH_OK) return rc;
+    trials++;
+    *ms = S->last_ms[0];
+    if (ref_st.empty()) ref_st = st;
+    for (int i = 0; i < ncells; i++)
+      if (st[i].cand != ref_st[i].cand || st[i].dfs_nodes != ref_st[i].dfs_nodes || st[i].leaves != ref_st[i].leaves ||
+          st[i].hall_ok != ref_st[i].hall_ok) {
+        set_error("autotune: stream results differ between launch settings");
+        return WZFH_EINTERNAL;
+      }
+    return WZFH_OK;
+  };
+  double best_ms = 0;
+  int rc = stream_trial(best, &best_ms);
+  if (rc != WZFH_OK) { S->tune = best; return rc; }
+  const std::vector<int> moves = {4, 8, 16, 32, 64}, hops = {0, 1, 2}, wins = {2, 4, 8, 16}, sleeps = {0, 16, 64, 128, 256};
+  for (int knob = 0; knob < 4; knob++) {
+    const std::vector<int> &vals = knob == 0 ? moves : knob == 1 ? hops : knob == 2 ? wins : sleeps;
+    for (int v : vals) {
+      wzfh_tuning t = best;
+      int &field = knob == 0 ? t.inner_moves : knob == 1 ? t.donate_hops : knob == 2 ? t.static_window : t.idle_sleep;
+      if (field == v) continue;
+      field = v;
+      double ms = 0;
+      rc = stream_trial(t, &ms);
+      if (rc != WZFH_OK) { S->tune = best; return rc; }
+      if (ms < best_ms) { best_ms = ms; best = t; }
+    }
+  }
+  // completer split depth, on the sample's own top-K candidates
+  wzfh_search_opts so = *opts;
+  so.batched = 1;
+  std::vector<wzfh_search_result> ref_res;
+  double best_c = -1;
+  for (int ext : {best.complete_split, 2, 3, 4}) {
+    if (best_c >= 0 && ext == best.complete_split) continue;
+    wzfh_tuning t = best;
+    t.complete_split = ext;
+    S->tune = t;
+    std::vector<wzfh_search_result> res(ncells);
+    rc = wzfh_search_cells(p, cells, ncells, &so, res.data());
+    if (rc != WZFH_OK && rc != WZFH_EINTERNAL) { S->tune = best; return rc; }
+    trials++;
+    const double ms = S->last_ms[1];
+    if (ref_res.empty()) ref_res = res;
+    for (int i = 0; i < ncells; i++)
+      if (res[i].hit != ref_res[i].hit || res[i].ab_nodes != ref_res[i].ab_nodes || res[i].A != ref_res[i].A ||
+          res[i].B != ref_res[i].B || res[i].clean_no != ref_res[i].clean_no || res[i].aborted != ref_res[i].aborted) {
+        S->tune = best;
+        set_error("autotune: completer results differ between split depths");
+        return WZFH_EINTERNAL;
+      }
+    if (best_c < 0 || ms < best_c) { best_c = ms; best.complete_split = ext; }
+  }
+  best.stream_ms = best_ms;
+  best.complete_ms = best_c;
+  best.trials = trials;
+  S->tune = best;
+  if (out) *out = best;
   return WZFH_OK;
 }
 

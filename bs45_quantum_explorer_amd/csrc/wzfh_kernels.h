@@ -74,6 +74,7 @@ struct StreamArgs {
   uint32_t static_win;
   int scan_max;                   // longest backoff (loop iterations) of idle polling
   int inner_moves;                // DFS moves a lane makes per pass of the kernel's main loop
+  int idle_sleep;                 // s_sleep count of a wave with nothing to walk (0 = none)
   uint32_t *cell_wp;              // first item not yet walked
   unsigned long long *cell_pn, *cell_pl, *cell_pc;   // walked node / leaf / candidate sums
   CandRec *cands;

@@ -872,6 +872,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
         if (__shfl(over, 0)) break;
       }
       __builtin_amdgcn_s_sleep(32);
+    } else if (qn == 0 && __ballot(st == ST_EXPAND || st == ST_BACK) == 0) {
+      // nothing to walk in this wave (waiting for work): yield the SIMD's issue slots to
+      // the waves that are walking
+      for (int i = 0; i < a.idle_sleep; i++) __builtin_amdgcn_s_sleep(1);
     }
   }
   if (telem && lane == 0) atomicAdd(&s_tel[1], (unsigned long long)(clock64() - t_start));

@@ -165,6 +165,31 @@ typedef struct {
 int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts,
                       wzfh_search_result *out);
 
+/* ---- Autotuning ----------------------------------------------------------------
+ * No reference counterpart. Launch settings that change only speed, never results: the
+ * stream kernel's DFS moves per lane per pass, the capped launches' donation generations
+ * and static-item window, and the completer's split depth. wzfh_autotune times them on a
+ * sample of cells with the caller's search options (coordinate descent, one knob at a
+ * time), checks that every trial returns the same per-cell results as the first (else
+ * WZFH_EINTERNAL), keeps the fastest in the problem handle and returns it. Environment
+ * variables (WZFH_INNER_MOVES, WZFH_DONATE_HOPS, WZFH_STATIC_WINDOW, WZFH_IDLE_SLEEP,
+ * WZFH_COMPLETE_SPLIT)
+ * still override the handle's settings. */
+typedef struct {
+  int inner_moves;        /* stream: DFS moves per lane per pass of the main loop (1..256) */
+  int donate_hops;        /* capped stream: generations of frontier donations (0..15)     */
+  int static_window;      /* capped stream: static items handed out ahead of the walk     */
+  int complete_split;     /* completer: quad-path levels per split round (0..4, 0 = none)  */
+  int idle_sleep;         /* stream: 64-cycle sleeps of a wave with nothing to walk (0..256)*/
+  int trials;             /* wzfh_autotune: configurations timed                           */
+  double stream_ms;       /* wzfh_autotune: stream kernel time of the chosen settings      */
+  double complete_ms;     /* wzfh_autotune: completer time of the chosen settings          */
+} wzfh_tuning;
+
+int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts, wzfh_tuning *out);
+int wzfh_get_tuning(const wzfh_problem *p, wzfh_tuning *out);
+int wzfh_set_tuning(wzfh_problem *p, const wzfh_tuning *t);
+
 /* ---- Serving: continuous batching of cell-search requests ------------------------
  * No reference counterpart (the reference runs fixed arm processes). Requests name a
  * problem (an initialized (n, class) handle, kept device-resident) and a cell; one worker

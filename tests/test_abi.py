@@ -48,6 +48,9 @@ def test_host_only_handle_refuses_device_calls():
     with pytest.raises(wz.WzfhError) as e:
         P.stream([0])
     assert e.value.code == wz.ENODEV
+    with pytest.raises(wz.WzfhError) as e:
+        P.tuning()
+    assert e.value.code == wz.ENODEV
 
 
 def test_host_npaf_and_flat_score_helpers():
