@@ -150,7 +150,7 @@ def main():
     ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')
     ap.add_argument('--max-batch', type=int, default=4096)
     ap.add_argument('--seed', type=int, default=44)
-    ap.add_argument('--cpu-cells', type=int, default=8, help='cells per class in the CPU baseline sample')
+    ap.add_argument('--cpu-cells', type=int, default=48, help='cells per class in the CPU baseline sample')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
@@ -289,7 +289,8 @@ def main():
 
 def cpu_baseline(a, cls, probs, cells):
     """The oracle port (oracle/wz_oracle.cpp) on the host cores: the same cell unit on a
-    bounded sample (the first --cpu-cells cells of each class), one cell per thread."""
+    bounded sample (--cpu-cells cells of each class, evenly spaced in its seeded sample), one cell
+    per thread."""
     sys.path.insert(0, str(ROOT / 'tests'))
     try:
         from oracle_lib import OracleProblem
