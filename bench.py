@@ -39,6 +39,7 @@ import bs45_quantum_explorer_amd as wz  # noqa: E402
 # MI355X figures used for the roofline. INT32 VALU peak from the published CU count and
 # clock: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (one 32-bit op per lane per cycle).
 INT_VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+HBM_PEAK_BPS = 8.0e12   # MI355X HBM3E
 OPS_PER_CD_NODE = 84          # per counted C,D DFS node (rec() entry) at m = 6
 
 
@@ -155,7 +156,12 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
     ap.add_argument('--tune-cells', type=int, default=64, help='cells of the autotuning sample')
+    ap.add_argument('--probe-traffic', action='store_true',
+                    help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + one stream launch)')
     a = ap.parse_args()
+    if a.probe_traffic:
+        probe_traffic(a)
+        return
     if a.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; cells shard by cell index mod G across GPUs '
                                    'with a final min over (rank, idx), which is not built here'}))
@@ -252,6 +258,10 @@ def main():
                 'unit': 'TOP/s (INT32 VALU)', 'frac': round(achieved / INT_VALU_PEAK_TOPS, 4), 'traffic': traffic,
                 'kernel': 'wzfh::stream_kernel',
                 'per_unit': f'{OPS_PER_CD_NODE} INT32 ops per counted C,D DFS node (rec() entry)',
+                'traffic_source': 'profiles/stream_kernel_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, '
+                                  'calibrated on a 256 MiB elementwise kernel)' if traffic else None,
+                'hbm_gb_per_s': round(traffic / (avg_ms / 1e3) / 1e9, 2) if traffic and avg_ms > 0 else None,
+                'hbm_frac': round(traffic / (avg_ms / 1e3) / HBM_PEAK_BPS, 5) if traffic and avg_ms > 0 else None,
                 'units_per_launch': int(units_per_launch), 'avg_launch_ms': round(avg_ms, 3)}
 
     cpu = None if a.no_cpu else cpu_baseline(a, cls, probs, cells)
@@ -285,6 +295,32 @@ def main():
     print(json.dumps(line), flush=True)
     for P in probs:
         P.close()
+
+
+def probe_traffic(a):
+    """For `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: first an elementwise kernel that
+    reads and writes exactly 256 MiB each (the counters' calibration), then one stream launch of
+    the bench workload (the first class, --cells-per-class seeded cells, the node budget) with the
+    bench's tuned launch settings when profiles/stream_kernel_traffic.json names them."""
+    import torch
+    x = torch.zeros(64 << 20, dtype=torch.int32, device='cuda')
+    y = torch.empty_like(x)
+    torch.add(x, 1, out=y)
+    torch.cuda.synchronize()
+    P = wz.Problem(a.n, classes(a.n)[0], canon=1, order=1)
+    cells = sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000)
+    tf = ROOT / 'profiles' / 'stream_kernel_traffic.json'
+    if tf.exists():
+        t = json.loads(tf.read_text()).get('tuning')
+        if t:
+            P.set_tuning(wz.Tuning(*[t.get(k, d) for k, d in (('inner_moves', 16), ('donate_hops', 1),
+                                                              ('static_window', 4), ('complete_split', 3),
+                                                              ('idle_sleep', 64), ('blocks_per_cu', 4))]))
+    st, _, _ = P.stream(cells, node_budget=a.node_budget, want_cands=False)
+    ms, units = P.last_kernel(0)
+    print(json.dumps({'calibration_bytes_each_way': 256 << 20, 'cells': len(cells), 'stream_ms': round(ms, 3),
+                      'counted_nodes': units}), flush=True)
+    P.close()
 
 
 def cpu_baseline(a, cls, probs, cells):

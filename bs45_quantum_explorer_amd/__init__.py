@@ -65,11 +65,11 @@ class SearchResult(C.Structure):
 class Tuning(C.Structure):
     """wzfh_tuning: launch settings that change speed only."""
     _fields_ = [('inner_moves', C.c_int), ('donate_hops', C.c_int), ('static_window', C.c_int),
-                ('complete_split', C.c_int), ('idle_sleep', C.c_int), ('trials', C.c_int),
-                ('stream_ms', C.c_double), ('complete_ms', C.c_double)]
+                ('complete_split', C.c_int), ('idle_sleep', C.c_int), ('blocks_per_cu', C.c_int),
+                ('trials', C.c_int), ('reserved', C.c_int), ('stream_ms', C.c_double), ('complete_ms', C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != 'reserved'}
 
 
 class ServeOpts(C.Structure):

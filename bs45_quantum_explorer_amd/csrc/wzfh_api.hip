@@ -61,7 +61,7 @@ struct DBuf {
 struct DeviceState {
   int device = 0, num_cu = 0;
   size_t complete_lds_max = 0;     // dynamic LDS bytes a completer block may use
-  wzfh_tuning tune{16, 1, 4, 3, 64, 0, 0.0, 0.0};   // launch settings (speed only)
+  wzfh_tuning tune{16, 1, 4, 3, 64, 4, 0, 0, 0.0, 0.0};   // launch settings (speed only)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // problem constants
@@ -442,7 +442,9 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     }
     // persistent grid: as many waves as can be resident (4 per SIMD at the kernel's
     // register budget); donation keeps them busy once the static items are gone
-    const int blocks = std::max(1, S->num_cu * 4);
+    int bpc = S->tune.blocks_per_cu;
+    if (const char *e = getenv("WZFH_BLOCKS_PER_CU")) bpc = atoi(e);
+    const int blocks = std::max(1, S->num_cu * std::max(1, std::min(4, bpc)));
     // idle polling backoff: waves per cell, so a few cells are not probed by thousands of
     // waves every iteration (the counters are device-scope atomics)
     a.scan_max = std::max(8, std::min(1024, (blocks * 4) / std::max(1, ncells)));
@@ -1026,7 +1028,7 @@ int wzfh_set_tuning(wzfh_problem *p, const wzfh_tuning *t) {
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
   if (t->inner_moves < 1 || t->inner_moves > 256 || t->donate_hops < 0 || t->donate_hops > 15 ||
       t->static_window < 1 || t->complete_split < 0 || t->complete_split > 4 || t->idle_sleep < 0 ||
-      t->idle_sleep > 256) {
+      t->idle_sleep > 256 || t->blocks_per_cu < 1 || t->blocks_per_cu > 4) {
     set_error("tuning value out of range");
     return WZFH_EINVAL;
   }
@@ -1065,12 +1067,13 @@ int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_sear
   double best_ms = 0;
   int rc = stream_trial(best, &best_ms);
   if (rc != WZFH_OK) { S->tune = best; return rc; }
-  const std::vector<int> moves = {4, 8, 16, 32, 64}, hops = {0, 1, 2}, wins = {2, 4, 8, 16}, sleeps = {0, 16, 64, 128, 256};
-  for (int knob = 0; knob < 4; knob++) {
-    const std::vector<int> &vals = knob == 0 ? moves : knob == 1 ? hops : knob == 2 ? wins : sleeps;
+  const std::vector<int> moves = {4, 8, 16, 32, 64}, hops = {0, 1, 2}, wins = {2, 4, 8, 16}, sleeps = {0, 16, 64, 128, 256}, bpcs = {1, 2, 3, 4};
+  for (int knob = 0; knob < 5; knob++) {
+    const std::vector<int> &vals = knob == 0 ? moves : knob == 1 ? hops : knob == 2 ? wins : knob == 3 ? sleeps : bpcs;
     for (int v : vals) {
       wzfh_tuning t = best;
-      int &field = knob == 0 ? t.inner_moves : knob == 1 ? t.donate_hops : knob == 2 ? t.static_window : t.idle_sleep;
+      int &field = knob == 0 ? t.inner_moves : knob == 1 ? t.donate_hops : knob == 2 ? t.static_window
+                 : knob == 3 ? t.idle_sleep : t.blocks_per_cu;
       if (field == v) continue;
       field = v;
       double ms = 0;

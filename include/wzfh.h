@@ -172,7 +172,7 @@ int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_
  * sample of cells with the caller's search options (coordinate descent, one knob at a
  * time), checks that every trial returns the same per-cell results as the first (else
  * WZFH_EINTERNAL), keeps the fastest in the problem handle and returns it. Environment
- * variables (WZFH_INNER_MOVES, WZFH_DONATE_HOPS, WZFH_STATIC_WINDOW, WZFH_IDLE_SLEEP,
+ * variables (WZFH_INNER_MOVES, WZFH_DONATE_HOPS, WZFH_STATIC_WINDOW, WZFH_IDLE_SLEEP, WZFH_BLOCKS_PER_CU,
  * WZFH_COMPLETE_SPLIT)
  * still override the handle's settings. */
 typedef struct {
@@ -181,7 +181,9 @@ typedef struct {
   int static_window;      /* capped stream: static items handed out ahead of the walk     */
   int complete_split;     /* completer: quad-path levels per split round (0..4, 0 = none)  */
   int idle_sleep;         /* stream: 64-cycle sleeps of a wave with nothing to walk (0..256)*/
+  int blocks_per_cu;      /* stream: persistent blocks (4 waves each) per compute unit (1..4)*/
   int trials;             /* wzfh_autotune: configurations timed                           */
+  int reserved;
   double stream_ms;       /* wzfh_autotune: stream kernel time of the chosen settings      */
   double complete_ms;     /* wzfh_autotune: completer time of the chosen settings          */
 } wzfh_tuning;
