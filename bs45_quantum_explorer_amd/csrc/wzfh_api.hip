@@ -61,11 +61,11 @@ struct DBuf {
 struct DeviceState {
   int device = 0, num_cu = 0;
   size_t complete_lds_max = 0;     // dynamic LDS bytes a completer block may use
-  wzfh_tuning tune{16, 1, 4, 3, 64, 4, 0, 0, 0.0, 0.0};   // launch settings (speed only)
+  wzfh_tuning tune{16, 1, 4, 3, 64, 2, 0, 0, 0.0, 0.0};   // launch settings (speed only)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // problem constants
-  DBuf hcos, hsin, order, rej, outer2, outer3, posidx, rows, kab, hall_frag;
+  DBuf hcos, hsin, order, rej, outer2, outer3, rows, kab, hall_frag;
   StreamArgs base{};                       // tables + constants of the stream kernel
   std::vector<uint8_t> angle_order;        // current evaluation order (angle j-1)
   std::vector<unsigned long long> angle_hist;
@@ -280,10 +280,9 @@ int device_init(wzfh_problem &P) {
   for (int j = 101; j <= 200; j++) S->angle_order.push_back((uint8_t)(j - 1));
   S->angle_hist.assign(200, 0);
   if (S->order.ensure(200) || S->rej.ensure(200 * sizeof(unsigned))) return WZFH_ENOMEM;
-  if (S->outer2.ensure(sizeof kOuter.t2) || S->outer3.ensure(sizeof kOuter.t3) || S->posidx.ensure(16)) return WZFH_ENOMEM;
+  if (S->outer2.ensure(sizeof kOuter.t2) || S->outer3.ensure(sizeof kOuter.t3)) return WZFH_ENOMEM;
   WZ_HIP(hipMemcpy(S->outer2.p, kOuter.t2, sizeof kOuter.t2, hipMemcpyHostToDevice));
   WZ_HIP(hipMemcpy(S->outer3.p, kOuter.t3, sizeof kOuter.t3, hipMemcpyHostToDevice));
-  WZ_HIP(hipMemcpy(S->posidx.p, kOuter.pos_index, 16, hipMemcpyHostToDevice));
   if (!P.rows.empty()) {
     if (S->rows.ensure(P.rows.size() * sizeof(AbRow))) return WZFH_ENOMEM;
     WZ_HIP(hipMemcpy(S->rows.p, P.rows.data(), P.rows.size() * sizeof(AbRow), hipMemcpyHostToDevice));
@@ -305,7 +304,7 @@ int device_init(wzfh_problem &P) {
 void device_free(wzfh_problem &P) {
   DeviceState *S = P.dev;
   if (!S) return;
-  for (DBuf *b : {&S->telem, &S->hall_frag, &S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->posidx, &S->rows, &S->kab,
+  for (DBuf *b : {&S->telem, &S->hall_frag, &S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->rows, &S->kab,
                   &S->cell_res, &S->cell_rootok, &S->cands, &S->ctr, &S->cell_tot, &S->kept,
                   &S->it_nodes, &S->it_noff, &S->it_loff, &S->it_coff, &S->it_leaves, &S->it_cands, &S->it_state,
                   &S->it_next, &S->it_key, &S->it_meta, &S->it_cell, &S->it_tail, &S->dq, &S->cell_lock,
