@@ -26,6 +26,8 @@ hipError_t launch_stream_finalize(const StreamArgs &a, uint64_t *cell_tot, unsig
 hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks, bool lds, size_t lds_bytes,
                            hipStream_t st);
 size_t complete_lds_bytes(int L, int words);
+hipError_t launch_complete_reduce(const CompleteArgs &a, int nent, int *fv, long long *fn, uint64_t *fA,
+                                  uint64_t *fB, int *fj, long long *fcc, long long *fcur, hipStream_t st);
 hipError_t complete_set_lds_limit(size_t bytes);
 
 static thread_local std::string g_err;
@@ -77,7 +79,7 @@ struct DeviceState {
   DBuf dq, cell_lock, cell_crossed, cell_wp, cell_pn, cell_pl, cell_pc, cell_open, cell_next, cell_sp;
   uint32_t epoch = 0;              // launch epoch tagging record states
   DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_icharged, c_ebase, c_epath, c_ebud,
-       c_scratch;
+       c_fv, c_fn, c_fA, c_fB, c_fj, c_fcc, c_fcur, c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
@@ -311,7 +313,7 @@ void device_free(wzfh_problem &P) {
                   &S->cell_crossed, &S->cell_wp, &S->cell_pn, &S->cell_pl, &S->cell_pc, &S->cell_open,
                   &S->cell_next, &S->cell_sp, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
                   &S->c_B, &S->c_charges, &S->c_icharged, &S->c_ebase, &S->c_epath, &S->c_ebud,
-                  &S->c_scratch})
+                  &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_fj, &S->c_fcc, &S->c_fcur, &S->c_scratch})
     b->release();
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -441,7 +443,9 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     }
     // persistent grid: as many waves as can be resident (4 per SIMD at the kernel's
     // register budget); donation keeps them busy once the static items are gone
-    int bpc = S->tune.blocks_per_cu;
+    // small launches (a few cells) walk each cell with more waves: 4 blocks per CU measured
+    // best for one cell (71-80 ms against 90-130 ms at 2); batches take the tuned value
+    int bpc = ncells <= 4 ? 4 : S->tune.blocks_per_cu;
     if (const char *e = getenv("WZFH_BLOCKS_PER_CU")) bpc = atoi(e);
     const int blocks = std::max(1, S->num_cu * std::max(1, std::min(4, bpc)));
     // idle polling backoff: waves per cell, so a few cells are not probed by thousands of
@@ -631,42 +635,77 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.item_charged = S->c_icharged.as<long long>();
   WZ_HIP(hipEventRecord(S->ev[2], S->stream));
   WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, lds, lds ? lds_bytes : 0, S->stream));
+  // per-entry results: on the device for split launches (one wave per entry), from the
+  // items themselves for unsplit ones; WZFH_CHECK_REDUCE=1 also scans the items on the host
+  // and compares
+  if (ext > 0) {
+    if (S->c_fv.ensure(nent * 4) || S->c_fn.ensure(nent * 8) || S->c_fA.ensure(nent * 8) || S->c_fB.ensure(nent * 8) ||
+        S->c_fj.ensure(nent * 4) || S->c_fcc.ensure(nent * 8) || S->c_fcur.ensure(nent * 8))
+      return WZFH_ENOMEM;
+    WZ_HIP(launch_complete_reduce(a, nent, S->c_fv.as<int>(), S->c_fn.as<long long>(), S->c_fA.as<uint64_t>(),
+                                  S->c_fB.as<uint64_t>(), S->c_fj.as<int>(), S->c_fcc.as<long long>(),
+                                  S->c_fcur.as<long long>(), S->stream));
+  }
   WZ_HIP(hipEventRecord(S->ev[3], S->stream));
-  std::vector<int> iv(nw);
-  std::vector<long long> in(nw), ic(nw);
-  std::vector<uint64_t> iA(nw), iB(nw);
-  WZ_HIP(hipMemcpyAsync(iv.data(), a.verdict, nw * 4, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(in.data(), a.nodes, nw * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(ic.data(), a.charges, nw * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(iA.data(), a.outA, nw * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipMemcpyAsync(iB.data(), a.outB, nw * 8, hipMemcpyDeviceToHost, S->stream));
-  WZ_HIP(hipStreamSynchronize(S->stream));
-  // Per-entry result: the items of an entry, in path order, are its DFS in order, so
-  // running sums of their counted and charged nodes give the reference's counts at every
-  // item boundary. The first item with a hit decides unless the budget ran out before it;
-  // otherwise the first item whose charges pass the budget holds the abort point (-1, with
-  // that item, the charges and counted nodes before it and its own charges).
-  const long long np = 1ll << (3 * ext);
-  for (int e = 0; e < nent; e++) {
-    long long cum_n = 0, cum_c = 0, cj = 0;
-    int vo = 2, jo = -1;
-    uint64_t Ao = 0, Bo = 0;
-    for (long long p = 0; p < np; p++) {
-      const size_t u = (size_t)((long long)e * np + p);
-      if (iv[u] == 1) { vo = 1; cum_n = 0; break; }
-      if (ext > 0 && (iv[u] == 3 || (budget > 0 && cum_c + ic[u] > ents[e].budget))) {
-        vo = -1; jo = (int)p; cj = ic[u];
-        break;
+  const bool check = ext > 0 && getenv("WZFH_CHECK_REDUCE") && atoi(getenv("WZFH_CHECK_REDUCE"));
+  std::vector<int> hv(nent), hj(nent);
+  std::vector<long long> hn(nent), hcc(nent), hcj(nent);
+  std::vector<uint64_t> hA(nent), hB(nent);
+  if (ext == 0 || check) {
+    std::vector<int> iv(nw);
+    std::vector<long long> in(nw), ic(nw);
+    std::vector<uint64_t> iA(nw), iB(nw);
+    WZ_HIP(hipMemcpyAsync(iv.data(), a.verdict, nw * 4, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(in.data(), a.nodes, nw * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(ic.data(), a.charges, nw * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(iA.data(), a.outA, nw * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(iB.data(), a.outB, nw * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipStreamSynchronize(S->stream));
+    // the same scan as the device reduce, on the host
+    const long long np = 1ll << (3 * ext);
+    for (int e = 0; e < nent; e++) {
+      long long cum_n = 0, cum_c = 0, cj = 0;
+      int vo = 2, jo = -1;
+      uint64_t Ao = 0, Bo = 0;
+      for (long long p = 0; p < np; p++) {
+        const size_t u = (size_t)((long long)e * np + p);
+        if (iv[u] == 1) { vo = 1; cum_n = 0; break; }
+        if (ext > 0 && (iv[u] == 3 || (budget > 0 && cum_c + ic[u] > ents[e].budget))) {
+          vo = -1; jo = (int)p; cj = ic[u];
+          break;
+        }
+        cum_n += in[u];
+        if (iv[u] == 0 || iv[u] == 3 || iv[u] == 4) {   // a hit, or (unsplit) an exactly counted abort
+          vo = iv[u]; Ao = iA[u]; Bo = iB[u];
+          break;
+        }
+        cum_c += ic[u];
       }
-      cum_n += in[u];
-      if (iv[u] == 0 || iv[u] == 3 || iv[u] == 4) {   // a hit, or (unsplit) an exactly counted abort
-        vo = iv[u]; Ao = iA[u]; Bo = iB[u];
-        break;
-      }
-      cum_c += ic[u];
+      hv[e] = vo; hn[e] = cum_n; hA[e] = Ao; hB[e] = Bo; hj[e] = jo; hcc[e] = cum_c; hcj[e] = cj;
     }
-    v[e] = vo; nd[e] = cum_n; A[e] = Ao; B[e] = Bo;
-    fj[e] = jo; fcc[e] = cum_c; fcur[e] = cj;
+  }
+  if (ext > 0) {
+    WZ_HIP(hipMemcpyAsync(v, S->c_fv.p, nent * 4, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(nd, S->c_fn.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(A, S->c_fA.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(B, S->c_fB.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(fj, S->c_fj.p, nent * 4, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(fcc, S->c_fcc.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(fcur, S->c_fcur.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipStreamSynchronize(S->stream));
+    if (check)
+      for (int e = 0; e < nent; e++) {
+        const bool same = v[e] == hv[e] && nd[e] == hn[e] && A[e] == hA[e] && B[e] == hB[e] &&
+                          (v[e] >= 0 || (fj[e] == hj[e] && fcc[e] == hcc[e] && fcur[e] == hcj[e]));
+        if (!same) {
+          set_error("completer: device reduce differs from the host scan");
+          return WZFH_EDEVICE;
+        }
+      }
+  } else {
+    for (int e = 0; e < nent; e++) {
+      v[e] = hv[e]; nd[e] = hn[e]; A[e] = hA[e]; B[e] = hB[e]; fj[e] = hj[e]; fcc[e] = hcc[e]; fcur[e] = hcj[e];
+    }
   }
   float t = 0;
   WZ_HIP(hipEventElapsedTime(&t, S->ev[2], S->ev[3]));

@@ -523,6 +523,77 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
 }  // namespace wzfh
 
 namespace wzfh {
+// Per-entry result of a split launch, one wavefront per entry: the items of an entry, in
+// path order, are its DFS in order, so running sums of their counted and charged nodes give
+// the reference's counts at every item boundary. The wave takes the items 64 at a time
+// (lane = item), forms prefix sums of charges and nodes with shuffles, and the first lane
+// with an event decides: pre-reject (1), a hit (0/4: nodes up to and including the item),
+// or the abort point (-1: the first item that stopped on its own or whose charges pass the
+// entry's budget; reported with the charges and nodes before it and its own charges).
+// Every branch is on a ballot, so the loop exits uniformly.
+__global__ __launch_bounds__(64) void complete_reduce_kernel(CompleteArgs a, int nent, int *fv, long long *fn,
+                                                             uint64_t *fA, uint64_t *fB, int *fj, long long *fcc,
+                                                             long long *fcur) {
+  const int e = blockIdx.x;
+  if (e >= nent) return;
+  const int lane = threadIdx.x & 63;
+  const long long np = 1ll << (3 * a.split_ext);
+  const bool limited = a.budget > 0;
+  const long long bud = a.ent_budget ? a.ent_budget[e] : a.budget;
+  long long cum_n = 0, cum_c = 0;   // before the current chunk (uniform)
+  int v_out = 2, j_out = -1;
+  long long n_out = 0, c_out = 0, cj_out = 0;
+  uint64_t A = 0, B = 0;
+  for (long long p0 = 0; p0 < np; p0 += 64) {
+    const long long p = p0 + lane;
+    const bool in = p < np;
+    const long long u = (long long)e * np + p;
+    const int v = in ? a.verdict[u] : 2;
+    const long long c = in ? a.charges[u] : 0, nd = in ? a.nodes[u] : 0;
+    // inclusive prefix sums over the chunk
+    long long pc = c, pn = nd;
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long tc = __shfl_up(pc, o), tn = __shfl_up(pn, o);
+      if (lane >= o) { pc += tc; pn += tn; }
+    }
+    const long long c_before = cum_c + pc - c, n_before = cum_n + pn - nd;
+    const bool ev = in && (v == 1 || v == 3 || v == 0 || v == 4 || (limited && c_before + c > bud));
+    const unsigned long long m = __ballot(ev);
+    if (m) {
+      const int f = __ffsll((long long)m) - 1;
+      const int vf = __shfl(v, f);
+      const long long cb = __shfl(c_before, f), nb = __shfl(n_before, f), cf = __shfl(c, f), nf = __shfl(nd, f);
+      if (vf == 1) { v_out = 1; n_out = 0; c_out = cb; }
+      else if (vf == 3 || (limited && cb + cf > bud)) { v_out = -1; j_out = (int)(p0 + f); n_out = nb; c_out = cb; cj_out = cf; }
+      else {
+        v_out = vf; n_out = nb + nf; c_out = cb;
+        A = __shfl(in ? a.outA[u] : 0ull, f);
+        B = __shfl(in ? a.outB[u] : 0ull, f);
+      }
+      break;
+    }
+    cum_c += __shfl(pc, 63);
+    cum_n += __shfl(pn, 63);
+    n_out = cum_n;
+    c_out = cum_c;
+  }
+  if (lane == 0) {
+    fv[e] = v_out;
+    fn[e] = n_out;
+    fA[e] = A;
+    fB[e] = B;
+    fj[e] = j_out;
+    fcc[e] = c_out;
+    fcur[e] = cj_out;
+  }
+}
+
+hipError_t launch_complete_reduce(const CompleteArgs &a, int nent, int *fv, long long *fn, uint64_t *fA,
+                                  uint64_t *fB, int *fj, long long *fcc, long long *fcur, hipStream_t st) {
+  hipLaunchKernelGGL(complete_reduce_kernel, dim3(nent), dim3(64), 0, st, a, nent, fv, fn, fA, fB, fj, fcc, fcur);
+  return hipGetLastError();
+}
+
 // Bytes of dynamic LDS one wave needs for row lists of `words` 64-row words.
 size_t complete_lds_bytes(int L, int words) {
   const size_t nlw = (size_t)(L / 2 + 1) * (size_t)words;

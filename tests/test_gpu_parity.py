@@ -134,6 +134,7 @@ def test_complete_split_abort_search_matches_reference(key, split, monkeypatch):
     last one walked whole for the exact abort count (split 0: every candidate unsplit)."""
     monkeypatch.setenv('WZFH_COMPLETE_SPLIT', str(split))
     monkeypatch.setenv('WZFH_COMPLETE_EXACT_ITEM', '1')
+    monkeypatch.setenv('WZFH_CHECK_REDUCE', '1')   # the device reduce against a host scan of the items
     _complete_group([c for c in GROUPS[key] if c['budget'] > 0])
 
 
