@@ -63,7 +63,7 @@ struct DBuf {
 struct DeviceState {
   int device = 0, num_cu = 0;
   size_t complete_lds_max = 0;     // dynamic LDS bytes a completer block may use
-  wzfh_tuning tune{16, 1, 4, 3, 64, 2, 0, 0, 0.0, 0.0};   // launch settings (speed only)
+  wzfh_tuning tune{16, 1, 4, 4, 64, 2, 0, 0, 0.0, 0.0};   // launch settings (speed only)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // problem constants
@@ -719,14 +719,14 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
 // one item that counts the abort exactly.
 static int complete_split_ext(const wzfh_problem &P) {
   int ext = (P.L1 / 2 >= 6) ? P.dev->tune.complete_split : 0;
-  if (const char *e = getenv("WZFH_COMPLETE_SPLIT")) ext = std::max(0, std::min(4, atoi(e)));
+  if (const char *e = getenv("WZFH_COMPLETE_SPLIT")) ext = std::max(0, std::min(6, atoi(e)));
   if (ext >= P.L1 / 2) ext = 0;
   return ext;
 }
 // an abort item this small (charges) is walked as one item; WZFH_COMPLETE_EXACT_ITEM overrides
 static long long exact_item_charges() {
   const char *e = getenv("WZFH_COMPLETE_EXACT_ITEM");
-  return e ? std::max(0ll, atoll(e)) : 16384;
+  return e ? std::max(0ll, atoll(e)) : 2048;   // a whole walk of 2048 charges is a few ms on one wave
 }
 
 static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
@@ -1065,7 +1065,7 @@ int wzfh_set_tuning(wzfh_problem *p, const wzfh_tuning *t) {
   if (!p || !t) return WZFH_EINVAL;
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
   if (t->inner_moves < 1 || t->inner_moves > 256 || t->donate_hops < 0 || t->donate_hops > 15 ||
-      t->static_window < 1 || t->complete_split < 0 || t->complete_split > 4 || t->idle_sleep < 0 ||
+      t->static_window < 1 || t->complete_split < 0 || t->complete_split > 6 || t->idle_sleep < 0 ||
       t->idle_sleep > 256 || t->blocks_per_cu < 1 || t->blocks_per_cu > 4) {
     set_error("tuning value out of range");
     return WZFH_EINVAL;
@@ -1125,7 +1125,7 @@ int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_sear
   so.batched = 1;
   std::vector<wzfh_search_result> ref_res;
   double best_c = -1;
-  for (int ext : {best.complete_split, 2, 3, 4}) {
+  for (int ext : {best.complete_split, 3, 4, 5}) {
     if (best_c >= 0 && ext == best.complete_split) continue;
     wzfh_tuning t = best;
     t.complete_split = ext;

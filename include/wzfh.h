@@ -179,7 +179,7 @@ typedef struct {
   int inner_moves;        /* stream: DFS moves per lane per pass of the main loop (1..256) */
   int donate_hops;        /* capped stream: generations of frontier donations (0..15)     */
   int static_window;      /* capped stream: static items handed out ahead of the walk     */
-  int complete_split;     /* completer: quad-path levels per split round (0..4, 0 = none)  */
+  int complete_split;     /* completer: quad-path levels per split round (0..6, 0 = none)  */
   int idle_sleep;         /* stream: 64-cycle sleeps of a wave with nothing to walk (0..256)*/
   int blocks_per_cu;      /* stream: persistent blocks (4 waves each) per compute unit (1..4)*/
   int trials;             /* wzfh_autotune: configurations timed                           */

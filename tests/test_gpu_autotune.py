@@ -25,7 +25,7 @@ def test_autotune_keeps_results_and_picks_valid_settings():
         t = P.autotune(cells, unit['top_k'], unit['budget'], node_budget=unit['node_budget'])
         assert t.trials >= 10 and t.stream_ms > 0
         assert t.inner_moves in (4, 8, 16, 32, 64) and t.donate_hops in (0, 1, 2)
-        assert t.static_window in (2, 4, 8, 16) and t.complete_split in (2, 3, 4) and t.idle_sleep in (0, 16, 64, 128, 256)
+        assert t.static_window in (2, 4, 8, 16) and t.complete_split in (3, 4, 5) and t.idle_sleep in (0, 16, 64, 128, 256)
         assert t.blocks_per_cu in (1, 2, 3, 4)
         got = P.tuning()
         assert (got.inner_moves, got.donate_hops, got.static_window, got.complete_split, got.idle_sleep,
