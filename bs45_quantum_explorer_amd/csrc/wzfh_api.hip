@@ -572,13 +572,18 @@ struct CompleteEntry {
 };
 
 static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff, const int *rlen,
-                           const std::vector<CompleteEntry> &ents, int maxrows, long long budget, int ext, int *v,
+                           const std::vector<CompleteEntry> &ents, int maxrows, long long budget, int ext,
+                           long long item_cap, int *v,
                            long long *nd, uint64_t *A, uint64_t *B, int *fj, long long *fcc, long long *fcur,
                            double *ms) {
   DeviceState *S = P.dev;
   const int nent = (int)ents.size();
   const int words = std::max(1, (maxrows + 63) / 64);
-  const long long nwork = (long long)nent << (3 * ext);
+  // first-round launches under the root canon enumerate depth-0 digits 6 and 7 only
+  bool root_pair = ext > 0 && !P.cfg.no_canon_ab;
+  for (const CompleteEntry &x : ents) root_pair = root_pair && x.base == 0;
+  const int items_log2 = 3 * ext - (root_pair ? 2 : 0);
+  const long long nwork = (long long)nent << items_log2;
   const int blocks = (int)std::max(1ll, std::min((long long)S->num_cu * 8, nwork));
   const size_t lds_bytes = complete_lds_bytes(P.L1, words);
   const bool lds = lds_bytes <= S->complete_lds_max;
@@ -612,7 +617,8 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.absA = std::abs(P.cfg.a); a.absB = std::abs(P.cfg.b);
   a.outer_k = P.cfg.outer_k; a.canon = !P.cfg.no_canon_ab;
   a.ncand = nent; a.budget = budget;
-  a.split_ext = ext; a.nwork = nwork;
+  a.split_ext = ext; a.nwork = nwork; a.root_pair = root_pair ? 1 : 0; a.items_log2 = items_log2;
+  a.item_cap = ext > 0 ? item_cap : 0;
   a.ent_path = S->c_epath.as<long long>(); a.ent_base = S->c_ebase.as<int>(); a.ent_budget = S->c_ebud.as<long long>();
   a.cd = S->c_cd.as<uint64_t>();
   a.row_off = S->c_roff.as<int>(); a.row_len = S->c_rlen.as<int>();
@@ -662,7 +668,7 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
     WZ_HIP(hipMemcpyAsync(iB.data(), a.outB, nw * 8, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipStreamSynchronize(S->stream));
     // the same scan as the device reduce, on the host
-    const long long np = 1ll << (3 * ext);
+    const long long np = 1ll << items_log2;
     for (int e = 0; e < nent; e++) {
       long long cum_n = 0, cum_c = 0, cj = 0;
       int vo = 2, jo = -1;
@@ -674,6 +680,7 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
           vo = -1; jo = (int)p; cj = ic[u];
           break;
         }
+        if (iv[u] == 5) { vo = -2; break; }   // capped before a decision
         cum_n += in[u];
         if (iv[u] == 0 || iv[u] == 3 || iv[u] == 4) {   // a hit, or (unsplit) an exactly counted abort
           vo = iv[u]; Ao = iA[u]; Bo = iB[u];
@@ -707,6 +714,12 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
       v[e] = hv[e]; nd[e] = hn[e]; A[e] = hA[e]; B[e] = hB[e]; fj[e] = hj[e]; fcc[e] = hcc[e]; fcur[e] = hcj[e];
     }
   }
+  if (root_pair)   // item index -> quad path digits below the entry
+    for (int e = 0; e < nent; e++)
+      if (fj[e] >= 0) {
+        const int lo = 3 * (ext - 1);
+        fj[e] = ((6 | (fj[e] >> lo)) << lo) | (fj[e] & ((1 << lo) - 1));
+      }
   float t = 0;
   WZ_HIP(hipEventElapsedTime(&t, S->ev[2], S->ev[3]));
   *ms += t;
@@ -724,6 +737,14 @@ static int complete_split_ext(const wzfh_problem &P) {
   return ext;
 }
 // an abort item this small (charges) is walked as one item; WZFH_COMPLETE_EXACT_ITEM overrides
+// first-round split items stop after this many charges (a few tens of ms on one wave); an
+// entry with such an item before its decision is split deeper instead. WZFH_COMPLETE_ITEM_CAP
+// overrides (0 = no cap).
+static long long item_charge_cap() {
+  const char *e = getenv("WZFH_COMPLETE_ITEM_CAP");
+  return e ? std::max(0ll, atoll(e)) : 16384;
+}
+
 static long long exact_item_charges() {
   const char *e = getenv("WZFH_COMPLETE_EXACT_ITEM");
   return e ? std::max(0ll, atoll(e)) : 2048;   // a whole walk of 2048 charges is a few ms on one wave
@@ -757,22 +778,28 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
   std::vector<CompleteEntry> pending(ncand);
   for (int i = 0; i < ncand; i++) pending[i] = CompleteEntry{i, 0, 0, budget, 0};
   // entries to split further, and entries walked as one item (exact abort count)
+  // entries split ext0 levels with a per-item charge cap, entries split deeper without a cap
+  // (an item hit the cap before the entry's decision), and entries walked as one item (exact
+  // abort count)
+  const int ext_deep = std::min(6, std::min(max_base, ext0 + 2));
+  const long long cap = item_charge_cap();
   std::vector<CompleteEntry> split = ext0 > 0 ? pending : std::vector<CompleteEntry>{};
-  std::vector<CompleteEntry> whole = ext0 > 0 ? std::vector<CompleteEntry>{} : pending;
-  while (!split.empty() || !whole.empty()) {
-    std::vector<CompleteEntry> next_split, next_whole;
-    for (int pass = 0; pass < 2; pass++) {
-      std::vector<CompleteEntry> &ents = pass == 0 ? split : whole;
-      const int ext = pass == 0 ? ext0 : 0;
-      const int chunk = std::max(1, (1 << 22) >> (3 * ext));
+  std::vector<CompleteEntry> deep, whole = ext0 > 0 ? std::vector<CompleteEntry>{} : pending;
+  while (!split.empty() || !deep.empty() || !whole.empty()) {
+    std::vector<CompleteEntry> next_split, next_deep, next_whole;
+    for (int pass = 0; pass < 3; pass++) {
+      std::vector<CompleteEntry> &ents = pass == 0 ? split : pass == 1 ? deep : whole;
+      const int ext = pass == 0 ? ext0 : pass == 1 ? ext_deep : 0;
+      const long long icap = pass == 0 && ext_deep > ext0 ? cap : 0;
+      const int chunk = std::max(1, (1 << 22) >> (3 * ext));   // at most 4M items per launch
       for (size_t c0 = 0; c0 < ents.size(); c0 += chunk) {
         const std::vector<CompleteEntry> part(ents.begin() + c0, ents.begin() + std::min(ents.size(), c0 + chunk));
         const int ne = (int)part.size();
         std::vector<int> v(ne), j(ne);
         std::vector<long long> nd(ne), cc(ne), cj(ne);
         std::vector<uint64_t> A(ne), B(ne);
-        int rc = complete_launch(P, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, v.data(), nd.data(),
-                                 A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms);
+        int rc = complete_launch(P, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, icap, v.data(),
+                                 nd.data(), A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms);
         if (rc != WZFH_OK) return rc;
         for (int e = 0; e < ne; e++) {
           const CompleteEntry &x = part[e];
@@ -783,6 +810,7 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
             fB[x.cand] = B[e];
             continue;
           }
+          if (v[e] == -2) { next_deep.push_back(x); continue; }   // the same entry, split deeper
           // the budget runs out inside item j: its subtree with the charges that remain
           CompleteEntry y{x.cand, (x.path << (3 * ext)) | j[e], x.base + ext, x.budget - cc[e], x.pre_nodes + nd[e]};
           if (cj[e] <= exact_item || y.base + ext0 > max_base) next_whole.push_back(y);
@@ -791,6 +819,7 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
       }
     }
     split.swap(next_split);
+    deep.swap(next_deep);
     whole.swap(next_whole);
   }
   S->last_ms[1] = ms;

@@ -131,10 +131,14 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
     const long long u = (long long)((((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(u_l >> 32))) << 32) |
                                     (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u_l));
     if (u >= a.nwork) break;
-    const int ci = (int)(u >> (3 * ext));                     // entry
-    const int sub = (int)(u & ((1ll << (3 * ext)) - 1));
+    const int ci = (int)(u >> a.items_log2);                  // entry
+    const int sub = (int)(u & ((1ll << a.items_log2) - 1));   // item of the entry
+    // quad path below the entry; with root_pair the depth-0 digit is 6 or 7 (root canon: every
+    // other depth-0 quad is cut before it is charged, so those paths carry no work)
+    const int psub = a.root_pair ? (((6 | (sub >> (3 * (ext - 1)))) << (3 * (ext - 1))) | (sub & ((1 << (3 * (ext - 1))) - 1)))
+                                 : sub;
     const int base = uni((a.ent_base ? a.ent_base[ci] : 0) + ext);   // levels replayed along the path
-    const long long path = uni64(((a.ent_path ? a.ent_path[ci] : 0ll) << (3 * ext)) | sub);   // depth l: bits 3(base-1-l)
+    const long long path = uni64(((a.ent_path ? a.ent_path[ci] : 0ll) << (3 * ext)) | psub);   // depth l: bits 3(base-1-l)
     // a budgeted launch (a.budget > 0) gives each entry the charges left to it, possibly 0
     const bool limited = a.budget > 0;
     const long long bud = uni64(a.ent_budget ? a.ent_budget[ci] : a.budget);
@@ -180,6 +184,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
       int st_k = 0, st_mask = 0, st_ties = 0, st_sums = 0;   // lane l = level l records
       Partners pp{0, 0, 0, 0, 0};
       bool pp_ok = false;         // pp holds the current node's partner sums
+      bool cap_stop = false;      // stopped at the launch's per-item charge cap
       // split items report their charges 256 at a time; an item whose entry's earlier items
       // have together charged more than the budget lies wholly past the abort point and stops
       // (its result is never used). An item before or at the abort point never stops early.
@@ -365,6 +370,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
               if (limited && ++cur > bud) { state = S_ABORT; continue; }
             }
             if (charged_out(cur)) { state = S_ABORT; continue; }
+            if (a.item_cap > 0 && cur >= a.item_cap) { cap_stop = true; state = S_ABORT; continue; }
             if (!((vmask >> k) & 1u)) { state = S_EXHAUSTED; continue; }
           } else {
             // the reference's quad loop on masks: every quad of cmask reached is charged;
@@ -376,6 +382,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
             total++;
             if (limited && ++cur > bud) { state = S_ABORT; continue; }
             if (charged_out(cur)) { state = S_ABORT; continue; }
+            if (a.item_cap > 0 && cur >= a.item_cap) { cap_stop = true; state = S_ABORT; continue; }
             if (!((vmask >> k) & 1u)) continue;
           }
           kcur = k;
@@ -502,7 +509,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
           const unsigned cm = (unsigned)uni(__builtin_amdgcn_readlane(st_mask, l)) & 0xFFu;
           if (cm & (0xFFu << (kl + 1)) & 0xFFu) total++;
         }
-        verdict = 3;
+        verdict = cap_stop ? 5 : 3;
         A = B = 0;
       } else {
         verdict = 2;
@@ -528,16 +535,17 @@ namespace wzfh {
 // the reference's counts at every item boundary. The wave takes the items 64 at a time
 // (lane = item), forms prefix sums of charges and nodes with shuffles, and the first lane
 // with an event decides: pre-reject (1), a hit (0/4: nodes up to and including the item),
-// or the abort point (-1: the first item that stopped on its own or whose charges pass the
-// entry's budget; reported with the charges and nodes before it and its own charges).
-// Every branch is on a ballot, so the loop exits uniformly.
+// the abort point (-1: the first item that stopped on its own budget or whose charges pass
+// the entry's budget; reported with the charges and nodes before it and its own charges), or
+// an item stopped at the launch's charge cap before any of those (-2: the entry is run again
+// split deeper). Every branch is on a ballot, so the loop exits uniformly.
 __global__ __launch_bounds__(64) void complete_reduce_kernel(CompleteArgs a, int nent, int *fv, long long *fn,
                                                              uint64_t *fA, uint64_t *fB, int *fj, long long *fcc,
                                                              long long *fcur) {
   const int e = blockIdx.x;
   if (e >= nent) return;
   const int lane = threadIdx.x & 63;
-  const long long np = 1ll << (3 * a.split_ext);
+  const long long np = 1ll << a.items_log2;
   const bool limited = a.budget > 0;
   const long long bud = a.ent_budget ? a.ent_budget[e] : a.budget;
   long long cum_n = 0, cum_c = 0;   // before the current chunk (uniform)
@@ -557,7 +565,7 @@ __global__ __launch_bounds__(64) void complete_reduce_kernel(CompleteArgs a, int
       if (lane >= o) { pc += tc; pn += tn; }
     }
     const long long c_before = cum_c + pc - c, n_before = cum_n + pn - nd;
-    const bool ev = in && (v == 1 || v == 3 || v == 0 || v == 4 || (limited && c_before + c > bud));
+    const bool ev = in && (v == 1 || v == 3 || v == 5 || v == 0 || v == 4 || (limited && c_before + c > bud));
     const unsigned long long m = __ballot(ev);
     if (m) {
       const int f = __ffsll((long long)m) - 1;
@@ -565,6 +573,7 @@ __global__ __launch_bounds__(64) void complete_reduce_kernel(CompleteArgs a, int
       const long long cb = __shfl(c_before, f), nb = __shfl(n_before, f), cf = __shfl(c, f), nf = __shfl(nd, f);
       if (vf == 1) { v_out = 1; n_out = 0; c_out = cb; }
       else if (vf == 3 || (limited && cb + cf > bud)) { v_out = -1; j_out = (int)(p0 + f); n_out = nb; c_out = cb; cj_out = cf; }
+      else if (vf == 5) { v_out = -2; n_out = nb; c_out = cb; }   // capped before a decision: split deeper
       else {
         v_out = vf; n_out = nb + nf; c_out = cb;
         A = __shfl(in ? a.outA[u] : 0ull, f);

@@ -107,10 +107,13 @@ struct CompleteArgs {
   // ent_budget[e] (null: budget). Per-item results go to verdict/nodes/outA/outB/charges at u.
   // Inputs cd/row_off/row_len are indexed by entry.
   int split_ext;
+  int root_pair;                  // 1: the first digit only takes the root-canon quads 6 and 7 (x >> 3(ext-1) = 0, 1)
+  int items_log2;                 // log2 of items per entry: 3 split_ext, or 3 split_ext - 2 with root_pair
   long long nwork;
   const long long *ent_path;
   const int *ent_base;
   const long long *ent_budget;
+  long long item_cap;             // split: an item stops after this many charges (verdict 5; 0 = no cap)
   long long *charges;
   long long *item_charged;        // split: charges each item has reported so far (256 at a time)
   const uint64_t *cd;          // [ncand][2]

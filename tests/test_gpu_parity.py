@@ -126,13 +126,16 @@ def test_complete_only_matches_reference(key):
 BUDGETED = sorted(k for k in GROUPS if any(c['budget'] > 0 for c in GROUPS[k]))
 
 
-@pytest.mark.parametrize('split', [0, 2, 3])
+@pytest.mark.parametrize('split,cap', [(0, 0), (2, 0), (3, 0), (3, 64)])
 @pytest.mark.parametrize('key', BUDGETED, ids=lambda k: f'n{k[0]}-{k[1]}-k{k[2]}')
-def test_complete_split_abort_search_matches_reference(key, split, monkeypatch):
+def test_complete_split_abort_search_matches_reference(key, split, cap, monkeypatch):
     """Budgeted completions with the candidate split into 8^split quad-path items, the item
     holding a budget abort split again down to single items (threshold 1 charge), and the
-    last one walked whole for the exact abort count (split 0: every candidate unsplit)."""
+    last one walked whole for the exact abort count (split 0: every candidate unsplit). With a
+    per-item charge cap, an entry whose capped item comes before its decision is run again
+    split 2 levels deeper."""
     monkeypatch.setenv('WZFH_COMPLETE_SPLIT', str(split))
+    monkeypatch.setenv('WZFH_COMPLETE_ITEM_CAP', str(cap))
     monkeypatch.setenv('WZFH_COMPLETE_EXACT_ITEM', '1')
     monkeypatch.setenv('WZFH_CHECK_REDUCE', '1')   # the device reduce against a host scan of the items
     _complete_group([c for c in GROUPS[key] if c['budget'] > 0])
