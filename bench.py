@@ -155,7 +155,7 @@ def main():
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
-    ap.add_argument('--tune-cells', type=int, default=64, help='cells of the autotuning sample')
+    ap.add_argument('--tune-cells', type=int, default=128, help='cells of the autotuning sample')
     ap.add_argument('--probe-traffic', action='store_true',
                     help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + one stream launch)')
     a = ap.parse_args()
