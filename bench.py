@@ -155,7 +155,8 @@ def main():
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
-    ap.add_argument('--tune-cells', type=int, default=128, help='cells of the autotuning sample')
+    ap.add_argument('--tune-cells', type=int, default=0,
+                    help='cells of the autotuning sample (0: as many as a class batch, --cells-per-class)')
     ap.add_argument('--probe-traffic', action='store_true',
                     help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + one stream launch)')
     a = ap.parse_args()
@@ -181,7 +182,7 @@ def main():
     tune = None
     if not a.no_tune:
         t0 = time.time()
-        sample = sample_cells(probs[0].live_cells(), a.tune_cells, a.seed * 7919 + 1)
+        sample = sample_cells(probs[0].live_cells(), a.tune_cells or a.cells_per_class, a.seed * 7919 + 1)
         t = probs[0].autotune(sample, a.top_k, a.ab_budget, cand_cap=a.cand_cap, node_budget=a.node_budget)
         for P in probs[1:]:
             P.set_tuning(t)
