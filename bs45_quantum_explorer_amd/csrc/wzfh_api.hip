@@ -1117,11 +1117,13 @@ int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_sear
   auto stream_trial = [&](const wzfh_tuning &t, double *ms) -> int {
     S->tune = t;
     std::vector<wzfh_cell_stats> st(ncells);
-    StreamOut so;
-    int rc = run_stream(*p, cells, ncells, opts->stream, st.data(), so);
-    if (rc != WZFH_OK) return rc;
-    trials++;
-    *ms = S->last_ms[0];
+    for (int rep = 0; rep < 2; rep++) {   // best of two: launch times vary run to run
+      StreamOut so;
+      int rc = run_stream(*p, cells, ncells, opts->stream, st.data(), so);
+      if (rc != WZFH_OK) return rc;
+      trials++;
+      *ms = rep == 0 ? S->last_ms[0] : std::min(*ms, S->last_ms[0]);
+    }
     if (ref_st.empty()) ref_st = st;
     for (int i = 0; i < ncells; i++)
       if (st[i].cand != ref_st[i].cand || st[i].dfs_nodes != ref_st[i].dfs_nodes || st[i].leaves != ref_st[i].leaves ||
