@@ -1,0 +1,64 @@
+"""Runs kernel-logic checks on the CPU emulation of libwzfh (tests/emu/build.py) against the
+golden fixtures; prints one line per check and 'bad N' at the end. Invoked in a subprocess by
+tests/test_emu_kernels.py so the emulated library never mixes with the real one."""
+import hashlib
+import os
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent.parent))
+sys.path.insert(0, str(HERE.parent))
+import numpy as np  # noqa: E402
+
+import bs45_quantum_explorer_amd as wz  # noqa: E402
+wz.LIBPATH = HERE / '_build' / 'libwzfh_emu.so'
+from golden_data import complete_cases, stream_runs  # noqa: E402
+
+bad = 0
+
+
+def report(ok, *msg):
+    global bad
+    bad += 0 if ok else 1
+    print('ok ' if ok else 'BAD', *msg, flush=True)
+
+
+# completer: budget aborts found through split items, capped items re-split deeper, the
+# abort item walked whole; the device reduce checked against the host scan of the items
+cases = [c for c in complete_cases() if c['n'] == 13 and c['name'] in ('n13:5', 'n13:23', 'n13:25', 'n13:37')]
+P = wz.Problem(13, tuple(cases[0]['sig']), m6=0, thm211b=0, thm212=0, outer_k=cases[0]['outer_k'])
+os.environ['WZFH_CHECK_REDUCE'] = '1'
+os.environ['WZFH_COMPLETE_EXACT_ITEM'] = '1'
+plan = [(2, 0, c) for c in cases] + [(1, 4, c) for c in cases if (c['name'], c['budget']) in (('n13:5', 0), ('n13:23', 300))]
+for split, cap, c in plan:
+    os.environ['WZFH_COMPLETE_SPLIT'] = str(split)
+    os.environ['WZFH_COMPLETE_ITEM_CAP'] = str(cap)
+    if True:
+        cd = np.array([[wz.pack(c['C']), wz.pack(c['D'])]], dtype=np.uint64)
+        o = P.complete(cd, budget=c['budget'])[0]
+        ok = (o.verdict, o.nodes) == (c['r'], c['nodes'])
+        if ok and c['r'] == 0:
+            ok = list(wz.unpack(o.A, 14)) == c['A'] and list(wz.unpack(o.B, 14)) == c['B']
+        report(ok, 'complete', c['name'], 'budget', c['budget'], 'split', split, 'cap', cap, (o.verdict, o.nodes),
+               (c['r'], c['nodes']))
+P.close()
+
+# stream: per-cell counts and stream hash of small-n runs, with donation forced every 2 moves
+os.environ['WZFH_DONATE_MOVES'] = '2'
+os.environ['WZFH_DONATE_LOW'] = '1000000'
+os.environ['WZFH_DONATE_ANY'] = '1'
+for rec in [r for r in stream_runs() if r['n'] == 8][:1]:
+    S = wz.Problem(rec['n'], rec['sig'], m6=1, thm211b=1, thm212=1, canon=rec['canon'], rev=rec['rev'],
+                   endpos=rec['endpos'], order=rec['order'])
+    cells = [c[0] for c in rec['cells']][:4]
+    if cells:
+        stats, offs, cd = S.stream(cells)
+        for i, row in enumerate(rec['cells'][:4]):
+            part = cd[offs[i]:offs[i + 1]]
+            h = hashlib.sha1(''.join(wz.dump_lines(rec['n'], part)).encode()).hexdigest()
+            st = stats[i]
+            got = [row[0], st.cand, st.leaves, st.hall_ok, st.dfs_nodes, h]
+            report(got == list(row), 'stream n8', rec['sig'], 'cell', row[0])
+    S.close()
+print('bad', bad, flush=True)
