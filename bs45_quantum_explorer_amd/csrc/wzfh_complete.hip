@@ -1,7 +1,9 @@
 // wzfh_complete.hip -- A,B completer kernel for gfx950.
 //
-// Restates fh_complete_ab / fh_ab_search (src/solver/wz_match.cpp:1092-1341) with
-// one candidate per wavefront. The mirror-quad DFS control is wave-uniform (scalar
+// Restates fh_complete_ab / fh_ab_search (src/solver/wz_match.cpp:1092-1341) with one
+// work item per wavefront: a whole candidate, or (split launches) the subtree of a candidate
+// below one quad path, replayed from the root; items in path order concatenate to the
+// reference's DFS, and a reduce kernel combines them. The mirror-quad DFS control is wave-uniform (scalar
 // registers); lane s holds the shift-s state: the target T[s] = -CD[s], the partial
 // A,B autocorrelation Dab[s], and the remaining-pair bound Kab[d][s] from an LDS table.
 // A, B are packed 64-bit words (bit i set = element i is -1).

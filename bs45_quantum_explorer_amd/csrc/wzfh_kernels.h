@@ -99,7 +99,7 @@ struct StreamArgs {
 // A,B completer kernel arguments (fh_complete_ab, wz_match.cpp:1092-1341).
 struct CompleteArgs {
   int n, L, m, absA, absB, outer_k, canon;
-  int ncand;
+  int ncand;                   // entries of the launch
   long long budget;
   // Work items. Entry e (a candidate, or a subtree of one below a quad path) has 8^split_ext
   // items: item u = e * 8^split_ext + x walks the subtree below path (ent_path[e] << 3 ext) | x
