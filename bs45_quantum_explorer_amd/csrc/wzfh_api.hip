@@ -463,6 +463,7 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     if (const char *e = getenv("WZFH_INNER_MOVES")) a.inner_moves = std::max(1, std::min(256, atoi(e)));
     a.don_low = (uint32_t)(blocks * 256 / 4);
     if (const char *e = getenv("WZFH_DONATE_MOVES")) a.don_moves = a.don_moves_front = (uint32_t)std::max(1, atoi(e));
+    if (const char *e = getenv("WZFH_DONATE_FRONT")) a.don_moves_front = (uint32_t)std::max(1, atoi(e));
     if (const char *e = getenv("WZFH_DONATE_HOPS")) a.don_hops = (uint32_t)std::min(15, std::max(0, atoi(e)));
     a.don_any = (getenv("WZFH_DONATE_ANY") && atoi(getenv("WZFH_DONATE_ANY"))) ? 1 : 0;
     if (const char *e = getenv("WZFH_DONATE_LOW")) a.don_low = (uint32_t)std::max(0, atoi(e));
