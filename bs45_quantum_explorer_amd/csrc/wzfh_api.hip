@@ -754,6 +754,19 @@ static long long exact_item_charges() {
   return e ? std::max(0ll, atoll(e)) : 2048;   // a whole walk of 2048 charges is a few ms on one wave
 }
 
+// Cells of one search batch that fit in a 1/(2 nshare) share of the device's free memory:
+// per cell the stream's work records (16 K donated records of 68 B, the static ones) and
+// candidates, and the completer's split items (top_k candidates x items per candidate x 44 B).
+long long serve_cell_slots(wzfh_problem &P, int nshare, const wzfh_search_opts &o) {
+  size_t free_b = 0, total_b = 0;
+  if (!P.dev || hipSetDevice(P.dev->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+    return 1ll << 30;
+  const int ext = complete_split_ext(P);
+  const double items = ext > 0 ? (double)(1ll << (3 * ext)) : 1.0;
+  const double per_cell = 16384.0 * 72.0 + (double)std::max(1, o.top_k) * items * 44.0 + 65536.0;
+  return std::max(1ll, (long long)((double)free_b / (2.0 * std::max(1, nshare)) / per_cell));
+}
+
 static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
                         wzfh_completion *out) {
   DeviceState *S = P.dev;

@@ -76,6 +76,8 @@ struct StreamOut {
 };
 int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o, wzfh_cell_stats *stats,
                StreamOut &out);
+// Batch size (cells) of one problem's search that fits its share of free device memory.
+long long serve_cell_slots(wzfh_problem &P, int nshare, const wzfh_search_opts &o);
 int problem_init(const wzfh_config &cfg, bool ab_side, bool with_device, wzfh_problem **out);
 // Device side (wzfh_api.hip).
 int device_init(wzfh_problem &P);

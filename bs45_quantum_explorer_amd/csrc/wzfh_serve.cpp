@@ -1,11 +1,13 @@
 // wzfh_serve.cpp -- cell-search serving loop with continuous batching.
 //
-// The reference has no server: its lanes are fixed arm processes. This is
-// the build's serving form of the same cell unit (wzfh_search_cells): requests name a
+// The reference has no server: its lanes are fixed arm processes started by the lane driver
+// (cluster/deploy/cluster_firsthit_probe.sh:100-107). This is a serving form of the same
+// cell unit (wzfh_search_cells): requests name a
 // problem (an (n, class) handle, set up once and kept device-resident) and a cell.
 // One worker thread per problem repeatedly admits whatever requests of its problem have
-// arrived -- up to a slot limit sized by device memory -- and runs them as one persistent
-// stream launch plus one completer launch on the problem's own HIP stream. Requests that
+// arrived -- up to max_batch cells and to the batch that fits its share of free device
+// memory -- and runs them as one persistent stream launch plus the completer launches on the
+// problem's own HIP stream. Requests that
 // arrive while a batch is on the device join the next batch of their problem as soon as
 // the current one returns, and batches of different problems run concurrently on their
 // streams, so the device work of mixed-n traffic overlaps. Results are identical to
@@ -36,6 +38,7 @@ struct wzfh_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::vector<std::deque<ServeReq>> pending;   // per problem
+  std::vector<long long> mem_slots;             // per problem: cells per batch that fit in memory
   std::deque<wzfh_serve_result> done;
   std::vector<std::thread> workers;
   bool closing = false;
@@ -53,8 +56,8 @@ static void worker_loop(wzfh_server *S, int pidx) {
       std::unique_lock<std::mutex> lk(S->mu);
       S->cv_work.wait(lk, [&] { return S->closing || !S->pending[pidx].empty(); });
       if (S->pending[pidx].empty() && S->closing) return;
-      const int slots = std::max(1, S->opts.max_batch);
-      while (!S->pending[pidx].empty() && (int)batch.size() < slots) {
+      const long long slots = std::max(1ll, std::min((long long)S->opts.max_batch, S->mem_slots[pidx]));
+      while (!S->pending[pidx].empty() && (long long)batch.size() < slots) {
         batch.push_back(S->pending[pidx].front());
         S->pending[pidx].pop_front();
       }
@@ -95,6 +98,7 @@ int wzfh_serve_open(wzfh_problem **problems, int nproblems, const wzfh_serve_opt
   S->problems.assign(problems, problems + nproblems);
   S->opts = *opts;
   S->pending.resize(nproblems);
+  for (int i = 0; i < nproblems; i++) S->mem_slots.push_back(wzfh::serve_cell_slots(*problems[i], nproblems, opts->search));
   for (int i = 0; i < nproblems; i++) S->workers.emplace_back(worker_loop, S, i);
   *out = S;
   return WZFH_OK;
