@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 LIBPATH = PKG / 'libwzfh.so'
 CLIPATH = PKG / 'wzfh_match'
 
-OK, EINVAL, ENOMEM, EDEVICE, ENODEV, ECAPACITY, EINTERNAL, ETRUNC = 0, -1, -2, -3, -4, -5, -6, -7
+OK, EINVAL, ENOMEM, EDEVICE, ENODEV, ECAPACITY, EINTERNAL, ETRUNC, EKERNEL = 0, -1, -2, -3, -4, -5, -6, -7, -8
 HIT, PREREJECT, CLEAN_NO, ABORTED, INTERNAL = 0, 1, 2, 3, 4
 
 EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy', 'wzfh_last_error',
@@ -26,9 +26,10 @@ EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy',
 
 
 class WzfhError(RuntimeError):
-    def __init__(self, code, msg):
+    def __init__(self, code, msg, results=None):
         super().__init__(f'wzfh error {code}: {msg}')
         self.code = code
+        self.results = results   # per-item results of a call that wrote them (WZFH_EINTERNAL)
 
 
 class Config(C.Structure):
@@ -59,7 +60,8 @@ class SearchOpts(C.Structure):
 class SearchResult(C.Structure):
     _fields_ = [('stream', CellStats), ('completed', C.c_int), ('hit', C.c_int), ('hit_rank', C.c_longlong),
                 ('hit_idx', C.c_longlong), ('ab_nodes', C.c_longlong), ('clean_no', C.c_int), ('aborted', C.c_int),
-                ('A', C.c_uint64), ('B', C.c_uint64), ('C', C.c_uint64), ('D', C.c_uint64)]
+                ('A', C.c_uint64), ('B', C.c_uint64), ('C', C.c_uint64), ('D', C.c_uint64),
+                ('internal', C.c_int), ('reserved2', C.c_int)]
 
 
 class Tuning(C.Structure):
@@ -222,9 +224,10 @@ class Problem:
             co = np.ascontiguousarray(np.asarray(cell_of, dtype=np.int32))
         rc = lib().wzfh_complete(self.h, cd.ctypes.data, co.ctypes.data if co is not None else None, nc,
                                  budget, out)
-        if rc not in (OK, EINTERNAL):
-            _check(rc)
-        return [out[i] for i in range(nc)]
+        got = [out[i] for i in range(nc)]
+        if rc != OK:   # WZFH_EINTERNAL: verdict 4 somewhere (the reference's exit 5); results attached
+            raise WzfhError(rc, lib().wzfh_last_error().decode(), got if rc == EINTERNAL else None)
+        return got
 
     def search(self, cells, top_k, budget, cand_cap=0, node_budget=0, batched=True, split_depth=0, window=0):
         cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
@@ -232,9 +235,10 @@ class Problem:
         opts = SearchOpts(StreamOpts(cand_cap, node_budget, split_depth, window), top_k, int(batched), budget)
         res = (SearchResult * nc)()
         rc = lib().wzfh_search_cells(self.h, cells.ctypes.data, nc, C.byref(opts), res)
-        if rc not in (OK, EINTERNAL):
-            _check(rc)
-        return [res[i] for i in range(nc)]
+        got = [res[i] for i in range(nc)]
+        if rc != OK:   # WZFH_EINTERNAL: cells with internal = 1 (the reference's exit 5); results attached
+            raise WzfhError(rc, lib().wzfh_last_error().decode(), got if rc == EINTERNAL else None)
+        return got
 
     def autotune(self, cells, top_k, budget, cand_cap=0, node_budget=0):
         """Times the launch settings on these cells with this search unit and keeps the fastest

@@ -478,7 +478,7 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     if (dqh[5]) {
       set_error(dqh[5] == 2 ? "stream kernel met an invalid work record (internal error)"
                             : "stream kernel made no progress for 5 s and stopped (internal error)");
-      return WZFH_EINTERNAL;
+      return WZFH_EKERNEL;
     }
     float ms = 0;
     WZ_HIP(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
@@ -498,7 +498,7 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     for (int cs = 0; cs < ncells; cs++)
       if (tot[3 * cs] == kUnsetOff) {
         set_error("stream: an uncut cell has an unfinished work item");
-        return WZFH_EINTERNAL;
+        return WZFH_EKERNEL;
       }
     std::vector<KeptCand> kc(nk);
     if (nk) WZ_HIP(hipMemcpy(kc.data(), S->kept.p, nk * sizeof(KeptCand), hipMemcpyDeviceToHost));
@@ -1072,13 +1072,18 @@ int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_
     rc = run_complete(*p, cd.data(), cellof.data(), nk, opts->budget, comp.data());
     p->dev->last_ms[0] = stream_ms;
     if (rc != WZFH_OK && rc != WZFH_EINTERNAL) return rc;
+    // the reference's drain completes the sorted ranks in order and stops at the first hit
+    // (wz_match.cpp:2686) or internal error (:2591-2595): ranks past it are not counted.
+    // A cell's completions sit in rank order in the launch's arrays.
     for (int t = 0; t < nk; t++) {
       wzfh_search_result &r = res[owner[t]];
+      if (r.hit || r.internal) continue;
       r.completed++;
       r.ab_nodes += comp[t].nodes;
       if (comp[t].verdict == 2) r.clean_no++;
       if (comp[t].verdict == 3) r.aborted++;
-      if (comp[t].verdict == 0 && !r.hit) {
+      if (comp[t].verdict == 4) r.internal = 1;
+      if (comp[t].verdict == 0) {
         r.hit = 1; r.hit_rank = rank[t]; r.hit_idx = sidx[t];
         r.A = comp[t].A; r.B = comp[t].B; r.C = cd[2 * t]; r.D = cd[2 * t + 1];
       }
@@ -1086,17 +1091,28 @@ int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_
     return rc;
   };
   if (opts->batched) return one_batch(cells, ncells, out);
+  // plain path: one launch per cell; every cell is searched, and an error other than a
+  // failed recheck (EINTERNAL, flagged per cell) fails the whole call
   double sms = 0, cms = 0;
   long long su = 0, cu = 0;
+  int status = WZFH_OK;
+  std::string internal_msg;
   for (int i = 0; i < ncells; i++) {
     int rc = one_batch(cells + i, 1, out + i);
-    if (rc != WZFH_OK) return rc;
+    if (rc == WZFH_EINTERNAL) { status = rc; internal_msg = wzfh_last_error(); }
+    else if (rc != WZFH_OK) {
+      const std::string msg = wzfh_last_error();
+      for (int j = 0; j < ncells; j++) out[j] = wzfh_search_result{};
+      set_error(msg);
+      return rc;
+    }
     sms += p->dev->last_ms[0]; cms += p->dev->last_ms[1];
     su += p->dev->last_units[0]; cu += p->dev->last_units[1];
   }
   p->dev->last_ms[0] = sms; p->dev->last_ms[1] = cms;
   p->dev->last_units[0] = su; p->dev->last_units[1] = cu;
-  return WZFH_OK;
+  if (status != WZFH_OK) set_error(internal_msg);
+  return status;
 }
 
 
@@ -1146,7 +1162,7 @@ int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_sear
       if (st[i].cand != ref_st[i].cand || st[i].dfs_nodes != ref_st[i].dfs_nodes || st[i].leaves != ref_st[i].leaves ||
           st[i].hall_ok != ref_st[i].hall_ok) {
         set_error("autotune: stream results differ between launch settings");
-        return WZFH_EINTERNAL;
+        return WZFH_EKERNEL;
       }
     return WZFH_OK;
   };
@@ -1192,7 +1208,7 @@ int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_sear
           res[i].B != ref_res[i].B || res[i].clean_no != ref_res[i].clean_no || res[i].aborted != ref_res[i].aborted) {
         S->tune = best;
         set_error("autotune: completer results differ between split depths");
-        return WZFH_EINTERNAL;
+        return WZFH_EKERNEL;
       }
     if (best_c < 0 || ms < best_c) { best_c = ms; best.complete_split = ext; }
   }

@@ -38,6 +38,10 @@ extern "C" {
 #define WZFH_ECAPACITY  -5   /* caller buffer too small (required size reported)   */
 #define WZFH_EINTERNAL  -6   /* completer FOUND failed the NPAF recheck (exit 5)   */
 #define WZFH_ETRUNC     -7   /* cell list hit the 20M cap (reference exit 2)       */
+#define WZFH_EKERNEL    -8   /* a kernel's own consistency check failed (stream
+                                watchdog, invalid work record, unfinished item,
+                                results that differ between launch settings); no
+                                result of the call is valid                        */
 
 /* Verdicts of one A,B completion (wz_match.cpp:1295-1297, 1330-1335). */
 #define WZFH_HIT        0
@@ -142,7 +146,13 @@ int wzfh_complete(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int n
  * WZ_FH_DRAIN_TOP=top_k, WZ_FH_DRAIN_BATCHES=1; wz_match.cpp:2656-2714), batched over
  * many cells in one persistent launch. Per cell: the first cand_cap candidates (and
  * at most node_budget DFS nodes) are stable-sorted by flat_score, the first top_k
- * are completed with `budget` nodes each under the cell's profile rows. */
+ * are completed with `budget` nodes each under the cell's profile rows. The per-cell
+ * completion counters follow the reference's drain, which stops at the first hit
+ * (:2686): completed, ab_nodes, clean_no and aborted count the sorted ranks up to and
+ * including the lowest-ranked hit (ranks past it are computed on the device in the same
+ * launch and not counted). Returns WZFH_EINTERNAL when a completion's FOUND failed the
+ * exact NPAF recheck (the reference exits 5 there, :2591-2595): the results are written
+ * and the cells concerned carry internal = 1. Any other error leaves no result valid. */
 typedef struct {
   wzfh_stream_opts stream;
   int top_k;
@@ -160,6 +170,9 @@ typedef struct {
   long long ab_nodes;     /* total completer nodes of the cell                       */
   int clean_no, aborted;
   uint64_t A, B, C, D;    /* the hit (if any)                                        */
+  int internal;           /* 1 if a counted completion had verdict 4 (NPAF recheck
+                             of a FOUND failed): the cell's result is not valid      */
+  int reserved2;
 } wzfh_search_result;
 
 int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts,

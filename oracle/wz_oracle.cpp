@@ -414,8 +414,9 @@ struct Stream {
   int tx[8], ty[8], tot[8], px[8] = {0}, py[8] = {0}, placed[8] = {0};
   int X[64], Y[64];
   long long cap, nout = 0, leaves = 0, ok = 0, nodes = 0, node_budget = 0;
-  bool stop = false;
+  bool stop = false, budget_cut = false;
   int8_t *out; long long out_cap;
+  std::vector<int8_t> *grow = nullptr;   // when set, every candidate is appended here instead
   Stream(const Problem &p, const Cell &cell, long long cap_, int8_t *o, long long oc)
       : P(p), L(p.n), m(p.m), half(p.n / 2), cap(cap_), out(o), out_cap(oc) {
     for (int c = 0; c < m; c++) { tx[c] = cell.px[c]; ty[c] = cell.py[c]; tot[c] = class_count(L, c, m); }
@@ -443,7 +444,10 @@ struct Stream {
     leaves++;
     if (!hall()) return;
     ok++;
-    if (nout < out_cap && out) {
+    if (grow) {
+      for (int i = 0; i < L; i++) grow->push_back((int8_t)X[i]);
+      for (int i = 0; i < L; i++) grow->push_back((int8_t)Y[i]);
+    } else if (nout < out_cap && out) {
       int8_t *o = out + nout * 2 * L;
       for (int i = 0; i < L; i++) { o[i] = (int8_t)X[i]; o[L + i] = (int8_t)Y[i]; }
     }
@@ -452,7 +456,7 @@ struct Stream {
   }
   void rec(int d) {
     if (stop) return;
-    if (node_budget > 0 && nodes >= node_budget) { stop = true; return; }   // node index >= budget
+    if (node_budget > 0 && nodes >= node_budget) { stop = budget_cut = true; return; }   // node index >= budget
     nodes++;
     for (int c = 0; c < m; c++) {
       int rem = tot[c] - placed[c], dx = tx[c] - px[c], dy = ty[c] - py[c];
@@ -775,32 +779,55 @@ int wzo_npaf_at(const int8_t *A, const int8_t *B, int n1, const int8_t *C, const
   return c;
 }
 
-long long wzo_cell_front(void *h, int pi, long long buf_cap, int top_k, long long budget,
-                         int *verdicts, long long *nodes, long long *stream_idx,
-                         wzo_stream_stats *st) {
-  return wzo_cell_front_budget(h, pi, buf_cap, 0, top_k, budget, verdicts, nodes, stream_idx, st);
-}
-
-long long wzo_cell_front_budget(void *h, int pi, long long buf_cap, long long node_budget, int top_k,
-                                long long budget, int *verdicts, long long *nodes, long long *stream_idx,
-                                wzo_stream_stats *st) {
+// The front-only cell unit (the reference's drain, wz_match.cpp:2656-2714, with
+// WZ_FH_BUF_CAP = buf_cap, WZ_FH_DRAIN_TOP = top_k, WZ_FH_DRAIN_BATCHES = 1): stream the
+// first buf_cap candidates of cell pi (and, when node_budget > 0, only nodes with DFS index
+// below it), stable-sort them by flat score (:2654-2655), complete the sorted ranks in order
+// under the cell's profile rows and stop after the first hit (:2686, the drain's break).
+// Per rank r < res->completed: verdict, nodes, in-cell stream index and flat score.
+long long wzo_cell_search(void *h, int pi, long long buf_cap, long long node_budget, int top_k, long long budget,
+                          wzo_search_result *res, int *verdicts, long long *nodes, long long *stream_idx,
+                          long long *scores) {
   Problem &P = *(Problem *)h;
-  int n = P.n;
-  std::vector<int8_t> buf((size_t)buf_cap * 2 * n);
-  long long got = wzo_stream_cell_budget(h, pi, buf_cap, node_budget, buf.data(), buf_cap, st);
+  const int n = P.n;
+  std::vector<int8_t> buf;
+  Stream S(P, P.cells[pi], buf_cap, nullptr, 0);
+  S.grow = &buf;
+  S.node_budget = node_budget;
+  S.rec(0);
+  const long long got = S.nout;
+  std::memset(res, 0, sizeof *res);
+  res->cand = got; res->leaves = S.leaves; res->hall_ok = S.ok; res->dfs_nodes = S.nodes;
+  res->capped = (buf_cap > 0 && got >= buf_cap) || S.budget_cut;
+  res->budget_cut = S.budget_cut;
+  res->hit_rank = res->hit_idx = -1;
   std::vector<std::pair<long long, long long>> key(got);
   for (long long i = 0; i < got; i++) key[i] = {wzo_flat_score(n, &buf[i * 2 * n], &buf[i * 2 * n + n]), i};
   std::stable_sort(key.begin(), key.end(), [](auto &x, auto &y) { return x.first < y.first; });
-  long long stop = std::min<long long>(got, top_k);
+  const long long stop = std::min<long long>(got, top_k);
   std::vector<int8_t> A(n + 1), B(n + 1);
   for (long long r = 0; r < stop; r++) {
-    long long i = key[r].second;
+    const long long i = key[r].second;
     long long nd = 0;
-    verdicts[r] = wzo_complete(h, pi, &buf[i * 2 * n], &buf[i * 2 * n + n], budget, &nd, A.data(), B.data());
-    nodes[r] = nd;
+    const int v = wzo_complete(h, pi, &buf[i * 2 * n], &buf[i * 2 * n + n], budget, &nd, A.data(), B.data());
+    if (verdicts) verdicts[r] = v;
+    if (nodes) nodes[r] = nd;
     if (stream_idx) stream_idx[r] = i;
+    if (scores) scores[r] = key[r].first;
+    res->completed++;
+    res->ab_nodes += nd;
+    if (v == 1) res->pre_rejected++;
+    else if (v == 2) res->clean_no++;
+    else if (v == 3) res->aborted++;
+    else if (v == 4) res->internal = 1;
+    if (v == 4) break;   // the reference exits here (wz_match.cpp:2591-2595)
+    if (v == 0) {
+      res->hit = 1; res->hit_rank = r; res->hit_idx = i;
+      for (int t = 0; t <= n; t++) { res->A[t] = A[t]; res->B[t] = B[t]; }
+      for (int t = 0; t < n; t++) { res->C[t] = buf[i * 2 * n + t]; res->D[t] = buf[i * 2 * n + n + t]; }
+      break;
+    }
   }
-  for (long long r = stop; r < top_k; r++) { verdicts[r] = -1; nodes[r] = 0; if (stream_idx) stream_idx[r] = -1; }
   return got;
 }
 

@@ -68,19 +68,30 @@ long long wzo_flat_score(int n, const int8_t *C, const int8_t *D);
 /* Summed NPAF at shift s (wz_match.cpp:184-192). */
 int wzo_npaf_at(const int8_t *A, const int8_t *B, int n1, const int8_t *C, const int8_t *D,
                 int n2, int s);
-/* Front-only cell unit: stream the first `buf_cap` candidates of cell pi, stable-sort
- * them by flat score, complete the first `top_k` with `budget`. verdicts/nodes get
- * top_k entries (order = sorted rank); returns candidates streamed. */
-long long wzo_cell_front(void *h, int pi, long long buf_cap, int top_k, long long budget,
-                         int *verdicts, long long *nodes, long long *stream_idx,
-                         wzo_stream_stats *st);
-/* Same with a DFS node budget: only nodes with DFS index < node_budget are visited
- * (the serving policy's per-cell budget; the reference has no such cap). */
+/* Same as wzo_stream_cell with a DFS node budget: only nodes with DFS index < node_budget
+ * are visited (the serving policy's per-cell budget; the reference has no such cap). */
 long long wzo_stream_cell_budget(void *h, int pi, long long cap, long long node_budget, int8_t *out,
                                  long long out_cap, wzo_stream_stats *st);
-long long wzo_cell_front_budget(void *h, int pi, long long buf_cap, long long node_budget, int top_k,
-                                long long budget, int *verdicts, long long *nodes, long long *stream_idx,
-                                wzo_stream_stats *st);
+/* One front-only cell search (the unit of wzfh_search_cells). */
+typedef struct {
+  long long cand, leaves, hall_ok, dfs_nodes;   /* stream of the cell (prefix under the caps)  */
+  int capped, budget_cut;   /* a cap or budget cut the cell / the node budget did            */
+  int completed;            /* completions run, in sorted order, up to and including a hit  */
+  int hit;
+  long long hit_rank, hit_idx;   /* sorted rank and in-cell stream index of the hit (-1)    */
+  long long ab_nodes;       /* completer nodes of the completions run                       */
+  int pre_rejected, clean_no, aborted, internal;
+  int8_t A[64], B[64], C[64], D[64];   /* the hit (A,B: n+1 entries, C,D: n)              */
+} wzo_search_result;
+/* Front-only cell unit (drain of wz_match.cpp:2656-2714 with BUF_CAP = buf_cap, DRAIN_TOP =
+ * top_k, DRAIN_BATCHES = 1): stream the first buf_cap candidates of cell pi (0 = all; at most
+ * node_budget DFS nodes when > 0), stable-sort them by flat score, complete the sorted ranks
+ * < top_k in order with `budget` under the cell's profile rows, stopping after the first hit.
+ * Per completed rank: verdict, nodes, in-cell stream index, flat score (arrays of top_k, may
+ * be NULL). Returns the candidates streamed. */
+long long wzo_cell_search(void *h, int pi, long long buf_cap, long long node_budget, int top_k, long long budget,
+                          wzo_search_result *res, int *verdicts, long long *nodes, long long *stream_idx,
+                          long long *scores);
 /* A1 outer tables (wz_match.cpp:560-598) for test cross-checks. */
 void wzo_outer_tables(uint8_t *outer2 /*16*5*9*/, uint8_t *outer3 /*128*5*9*13*/);
 

@@ -22,6 +22,15 @@ class StreamStats(C.Structure):
                 ('cand', C.c_longlong)]
 
 
+class SearchResult(C.Structure):
+    """wzo_search_result: one front-only cell search (the unit of wzfh_search_cells)."""
+    _fields_ = [('cand', C.c_longlong), ('leaves', C.c_longlong), ('hall_ok', C.c_longlong),
+                ('dfs_nodes', C.c_longlong), ('capped', C.c_int), ('budget_cut', C.c_int), ('completed', C.c_int),
+                ('hit', C.c_int), ('hit_rank', C.c_longlong), ('hit_idx', C.c_longlong), ('ab_nodes', C.c_longlong),
+                ('pre_rejected', C.c_int), ('clean_no', C.c_int), ('aborted', C.c_int), ('internal', C.c_int)] + \
+               [(k, C.c_int8 * 64) for k in 'ABCD']
+
+
 _lib = None
 
 
@@ -47,17 +56,13 @@ def lib():
         L.wzo_flat_score.restype = C.c_longlong
         L.wzo_flat_score.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
         L.wzo_npaf_at.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
-        L.wzo_cell_front.restype = C.c_longlong
-        L.wzo_cell_front.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_int, C.c_longlong,
-                                     C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(StreamStats)]
         L.wzo_outer_tables.argtypes = [C.c_void_p, C.c_void_p]
         L.wzo_stream_cell_budget.restype = C.c_longlong
         L.wzo_stream_cell_budget.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_void_p,
                                              C.c_longlong, C.POINTER(StreamStats)]
-        L.wzo_cell_front_budget.restype = C.c_longlong
-        L.wzo_cell_front_budget.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_int,
-                                            C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p,
-                                            C.POINTER(StreamStats)]
+        L.wzo_cell_search.restype = C.c_longlong
+        L.wzo_cell_search.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.c_longlong,
+                                      C.POINTER(SearchResult), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -112,14 +117,18 @@ class OracleProblem:
         got = lib().wzo_stream_cell_budget(self.h, pi, cap, node_budget, out.ctypes.data, out_cap, C.byref(st))
         return got, out[:min(got, out_cap)], st
 
-    def front_budget(self, pi, buf_cap, node_budget, top_k, budget):
-        v = np.zeros(top_k, np.int32)
-        nd = np.zeros(top_k, np.int64)
-        idx = np.zeros(top_k, np.int64)
-        st = StreamStats()
-        got = lib().wzo_cell_front_budget(self.h, pi, buf_cap, node_budget, top_k, budget, v.ctypes.data,
-                                          nd.ctypes.data, idx.ctypes.data, C.byref(st))
-        return got, v, nd, idx, st
+    def search(self, pi, buf_cap, node_budget, top_k, budget):
+        """Front-only cell search (wzo_cell_search): returns (result, per-rank arrays) with
+        per-rank verdict, nodes, in-cell stream index and flat score of the completions run."""
+        v = np.zeros(max(top_k, 1), np.int32)
+        nd = np.zeros(max(top_k, 1), np.int64)
+        idx = np.zeros(max(top_k, 1), np.int64)
+        sc = np.zeros(max(top_k, 1), np.int64)
+        res = SearchResult()
+        lib().wzo_cell_search(self.h, pi, buf_cap, node_budget, top_k, budget, C.byref(res), v.ctypes.data,
+                              nd.ctypes.data, idx.ctypes.data, sc.ctypes.data)
+        k = res.completed
+        return res, {'verdict': v[:k], 'nodes': nd[:k], 'idx': idx[:k], 'score': sc[:k]}
 
     def complete(self, C_, D_, budget, pi=-1):
         n = self.n
@@ -130,15 +139,6 @@ class OracleProblem:
         r = lib().wzo_complete(self.h, pi, Cc.ctypes.data, Dc.ctypes.data, budget, C.byref(nodes),
                                A.ctypes.data, B.ctypes.data)
         return r, nodes.value, A, B
-
-    def front(self, pi, buf_cap, top_k, budget):
-        v = np.zeros(top_k, np.int32)
-        nd = np.zeros(top_k, np.int64)
-        idx = np.zeros(top_k, np.int64)
-        st = StreamStats()
-        got = lib().wzo_cell_front(self.h, pi, buf_cap, top_k, budget, v.ctypes.data, nd.ctypes.data,
-                                   idx.ctypes.data, C.byref(st))
-        return got, v, nd, idx, st
 
 
 def flat_score(n, C_, D_):
