@@ -1,24 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark: n = 44 cell searches per second on one MI355X, served vs plain GPU.
 
-Workload (BS(45,44), the reference's n = 44 production setting): the 12 signature classes
-at n = 44 (or --classes), mod-6 cells with Thm 2.11b/2.12 (forced at n >= 36), orbit canon,
-flat order; cells drawn from each class's live list by a seeded splitmix64. One cell
-search is the front-only unit of the reference arm: stream the cell's C,D candidates in
-DFS order (at most --node-budget DFS nodes, a serving policy; the reference caps cells by
-candidates or wall time only), stable-sort them by flatness, complete the top --top-k with
---ab-budget A,B nodes each under the cell's profile rows. Inputs are synthetic in the sense
-that cells are sampled; every computation is the real one, results are exact.
+Workload (BS(45,44), the n = 44 configuration: 1,024 seeded cells from each of the 12
+signature classes, A,B completion budget 2^24). Classes at n = 44 with mod-6 cells, Thm
+2.11b/2.12 (forced at n >= 36), orbit canon, flat order; each class's 1,024 cells are drawn
+from its live list by a seeded splitmix64. One cell search is the front-only unit of the
+reference arm: stream the cell's C,D candidates in DFS order, stable-sort them by flatness,
+complete the flattest --top-k (32; the configuration names no width, 32 is the serving
+width) with the A,B budget each under the cell's profile rows, stopping at the first hit.
+n = 44 cells hold ~10^11 DFS nodes each, so every cell is cut at --node-budget DFS nodes
+(2^22): a policy of this build (the reference caps cells by candidates or wall time only),
+labelled in config.workload. Inputs are synthetic in the sense that cells are sampled;
+every computation is the real one, results are exact.
 
-A step = one pass of the hot path over the batch of sampled cells, served: every cell is
-submitted to the continuous-batching server (one worker and HIP stream per class, requests
-of a class admitted together) and the step ends when every result is back. `value` is
-served cells/s. Beside it on the same JSON line:
-  plain    one cell per search call (the single-cell GPU path), cells/s and latency
-  batched  one persistent launch per class (the kernel-level view; roofline source)
-  latency  open-loop Poisson arrivals at --load x the served rate: p50 / p99 cell latency
-  roofline the stream kernel's INT32-VALU rate on counted DFS nodes, against peak
-  cpu      the oracle port on the host cores on a bounded sample of the same cells
+A step = one slice of that set: slice k holds cells j = k (mod 12) of every class's seeded
+sample (~1,024 cells across the 12 classes), so 12 consecutive steps search the whole
+12 x 1,024 set once; step s (warmup steps first) takes slice s mod 12. The step is served:
+every cell is submitted to the continuous-batching server and the step ends when every
+result is back. `value` is served cells/s over the timed steps. On the same JSON line:
+  plain     one cell per search call (the single-cell GPU path), cells/s and latency
+  batched   one persistent launch per class on slice 0 (the kernel-level view)
+  latency   open-loop Poisson arrivals at --load x the served rate: p50 / p99 cell latency
+  roofline  the dominant kernel's rate over the timed steps (HIP-event time of its launches,
+            summed over the classes' handles) against the INT32-VALU peak; both kernels under
+            rooflines{}
+  verify    the matrix-core leaf test's false rejects / accepts against FP64 on a sample
+  cpu       the oracle port on the host cores on a bounded sample of the same cells, with its
+            per-cell results compared to the served ones (mismatches must be 0)
 """
 import argparse
 import json
@@ -35,6 +43,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 import bs45_quantum_explorer_amd as wz  # noqa: E402
+from bs45_quantum_explorer_amd.workload import classes, sample_cells  # noqa: E402
 
 # MI355X figures used for the roofline. INT32 VALU peak from the published CU count and
 # clock: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (one 32-bit op per lane per cycle).
@@ -43,39 +52,10 @@ HBM_PEAK_BPS = 8.0e12   # MI355X HBM3E
 OPS_PER_CD_NODE = 84          # per counted C,D DFS node (rec() entry) at m = 6
 
 
-def classes(n):
-    """Signature classes (a <= b, c <= d) with a^2+b^2+c^2+d^2 = 4n+2 and the parities of
-    lengths n+1 (A,B) and n (C,D)."""
-    out = []
-    for a in range(n + 2):
-        for b in range(a, n + 2):
-            for c in range(n + 1):
-                for d in range(c, n + 1):
-                    if (a - n - 1) % 2 or (b - n - 1) % 2 or (c - n) % 2 or (d - n) % 2:
-                        continue
-                    if a * a + b * b + c * c + d * d == 4 * n + 2:
-                        out.append((a, b, c, d))
-    return out
-
-
-def splitmix64(state):
-    while True:
-        state = (state + 0x9E3779B97F4A7C15) & (2 ** 64 - 1)
-        z = state
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2 ** 64 - 1)
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2 ** 64 - 1)
-        yield z ^ (z >> 31)
-
-
-def sample_cells(live, k, seed):
-    """k distinct cells from `live`, chosen by splitmix64 (partial Fisher-Yates)."""
-    live = list(live)
-    rng = splitmix64(seed)
-    k = min(k, len(live))
-    for i in range(k):
-        j = i + next(rng) % (len(live) - i)
-        live[i], live[j] = live[j], live[i]
-    return sorted(live[:k])
+def ops_per_ab_node(n):
+    """Per counted completer node: two O(L) placements updating Dab and Kab plus the n-shift
+    bound scan, 6L + n INT ops (314 at n = 44)."""
+    return 6 * (n + 1) + n
 
 
 def log(msg):
@@ -128,10 +108,42 @@ def poisson_latency(server, reqs, rate, count, seed):
             next_t += rng.expovariate(rate)
         wait_ms = int(max(0.0, min(50.0, (next_t - now) * 1e3))) if i < count else 2000
         for r in server.poll(max_results=4096, min_results=1, timeout_ms=wait_ms):
+            if r.status != 0:
+                raise RuntimeError(f'served request failed with status {r.status}')
             pending.discard(r.ticket)
             lat.append(r.finish_s - r.submit_s)
             done += 1
     return lat, time.perf_counter() - t0
+
+
+def kernel_roofline(name, ops_per_unit, per_unit, ms, units, launches, wall_s, traffic=None):
+    """Achieved INT32 rate of one kernel: algorithmic ops of its launches over their summed
+    HIP-event durations (= per-launch ops / average launch duration)."""
+    avg_ms = ms / launches if launches else 0.0
+    achieved = ops_per_unit * units / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+    out = {'bound': 'valu', 'achieved': round(achieved, 4), 'peak': round(INT_VALU_PEAK_TOPS, 2),
+           'unit': 'TOP/s (INT32 VALU)', 'frac': round(achieved / INT_VALU_PEAK_TOPS, 5),
+           'traffic': traffic.get('bytes_per_launch') if traffic else None,
+           'kernel': name, 'per_unit': per_unit, 'units_timed': int(units), 'launches_timed': int(launches),
+           'avg_launch_ms': round(avg_ms, 3), 'units_per_launch': round(units / launches, 1) if launches else 0,
+           # the same ops over the timed steps' wall clock (launches of several classes overlap)
+           'device_rate_tops': round(ops_per_unit * units / wall_s / 1e12, 4) if wall_s > 0 else None}
+    if traffic:
+        out['traffic_source'] = traffic.get('source')
+        if avg_ms > 0 and traffic.get('bytes_per_launch'):
+            out['hbm_gb_per_s'] = round(traffic['bytes_per_launch'] / (avg_ms / 1e3) / 1e9, 3)
+            out['hbm_frac'] = round(traffic['bytes_per_launch'] / (avg_ms / 1e3) / HBM_PEAK_BPS, 6)
+    return out
+
+
+def load_traffic():
+    tf = ROOT / 'profiles' / 'kernel_traffic_n44.json'
+    if not tf.exists():
+        return {}
+    try:
+        return json.loads(tf.read_text()).get('kernels', {})
+    except ValueError:
+        return {}
 
 
 def main():
@@ -141,228 +153,335 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--n', type=int, default=44)
     ap.add_argument('--classes', type=int, default=12, help='how many of the n=44 classes to use')
-    ap.add_argument('--cells-per-class', type=int, default=256)
+    ap.add_argument('--cells-per-class', type=int, default=1024)
+    ap.add_argument('--slices', type=int, default=12, help='a step searches every slices-th cell of each class')
     ap.add_argument('--node-budget', type=int, default=1 << 22)
     ap.add_argument('--cand-cap', type=int, default=0)
     ap.add_argument('--top-k', type=int, default=32)
-    ap.add_argument('--ab-budget', type=int, default=1 << 20)
+    ap.add_argument('--ab-budget', type=int, default=1 << 24)
     ap.add_argument('--plain-cells', type=int, default=2, help='cells per class timed one per call')
     ap.add_argument('--latency-requests', type=int, default=1500)
     ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')
     ap.add_argument('--max-batch', type=int, default=4096)
     ap.add_argument('--seed', type=int, default=44)
-    ap.add_argument('--cpu-cells', type=int, default=48, help='cells per class in the CPU baseline sample')
+    ap.add_argument('--verify-cells', type=int, default=32, help='cells per class in the FP64 verify sample')
+    ap.add_argument('--cpu-stride', type=int, default=2, help='CPU sample: every k-th cell of slice 0 per class')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
     ap.add_argument('--tune-cells', type=int, default=0,
-                    help='cells of the autotuning sample (0: as many as a class batch, --cells-per-class)')
+                    help='cells of the autotuning sample (0: one class slice, the batch a class launch sees)')
     ap.add_argument('--probe-traffic', action='store_true',
-                    help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + one stream launch)')
+                    help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + one batched search)')
+    ap.add_argument('--traffic-summary', nargs=2, metavar=('FETCH_CSV', 'WRITE_CSV'),
+                    help='summarize two rocprofv3 --pmc counter_collection.csv files into profiles/')
+    ap.add_argument('--serve-sweep', action='store_true',
+                    help='the serving configuration: Poisson requests with n ~ U{24..45}, offered rate swept to '
+                         'saturation (its own JSON line)')
+    ap.add_argument('--sweep-requests', type=int, default=600, help='requests per offered rate')
+    ap.add_argument('--sweep-ab-budget', type=int, default=1 << 20)
     a = ap.parse_args()
-    if a.probe_traffic:
-        probe_traffic(a)
+    if a.traffic_summary:
+        traffic_summary(*a.traffic_summary)
         return
     if a.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; cells shard by cell index mod G across GPUs '
                                    'with a final min over (rank, idx), which is not built here'}))
         sys.exit(2)
+    if a.probe_traffic:
+        probe_traffic(a)
+        return
+    if a.serve_sweep:
+        serve_sweep(a)
+        return
 
     t_setup = time.time()
     cls = classes(a.n)[:a.classes]
-    probs, cells = [], []
+    probs, samples = [], []
     for i, sig in enumerate(cls):
         P = wz.Problem(a.n, sig, canon=1, order=1)
         probs.append(P)
-        cells.append(sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000 + i))
+        samples.append(sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000 + i))
         log(f'setup class {i + 1}/{len(cls)} {sig}: {P.num_cells()} cells')
     setup_s = time.time() - t_setup
-    # launch settings (speed only): tuned once on a separate sample of the first class and
-    # applied to every class (the tuner checks that each setting returns identical results)
+    slices = [[(ci, c) for ci, smp in enumerate(samples) for c in smp[k::a.slices]] for k in range(a.slices)]
+    search = dict(cand_cap=a.cand_cap, node_budget=a.node_budget)
+    # launch settings (speed only): tuned once on the first class's cells of one slice (the batch
+    # a class launch sees) and applied to every class; the tuner checks that each setting
+    # returns identical results
     tune = None
     if not a.no_tune:
         t0 = time.time()
-        sample = sample_cells(probs[0].live_cells(), a.tune_cells or a.cells_per_class, a.seed * 7919 + 1)
-        t = probs[0].autotune(sample, a.top_k, a.ab_budget, cand_cap=a.cand_cap, node_budget=a.node_budget)
+        sample = sample_cells(probs[0].live_cells(), a.tune_cells or len(slices[0]) // max(1, len(probs)),
+                              a.seed * 7919 + 1)
+        t = probs[0].autotune(sample, a.top_k, a.ab_budget, **search)
         for P in probs[1:]:
             P.set_tuning(t)
         tune = dict(t.as_dict(), seconds=round(time.time() - t0, 1), sample_cells=len(sample))
         log(f'autotune: {tune}')
-    reqs = [(pi, c) for pi, cl in enumerate(cells) for c in cl]
-    total_cells = len(reqs)
-    log(f'setup: {len(cls)} classes, {total_cells} cells, {setup_s:.1f} s')
-    search = dict(cand_cap=a.cand_cap, node_budget=a.node_budget)
+    log(f'setup: {len(cls)} classes, {sum(map(len, samples))} cells in {a.slices} slices, {setup_s:.1f} s')
 
     # ---- served (the measured value)
+    served = {}
     server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=a.max_batch, **search)
     try:
         for w in range(a.warmup):
-            _, dt = served_pass(server, reqs)
-            log(f'served warmup {w + 1}/{a.warmup}: {dt * 1e3:.1f} ms')
-        times, served = [], None
+            reqs = slices[w % a.slices]
+            got, dt = served_pass(server, reqs)
+            served.update({rq: r.result for rq, r in zip(reqs, got)})
+            log(f'served warmup {w + 1}/{a.warmup} (slice {w % a.slices}): {len(reqs)} cells, {dt * 1e3:.1f} ms')
+        for P in probs:
+            P.kernel_totals(reset=True)
+        times, ncells, res_timed = [], 0, []
+        t_timed = time.perf_counter()
         for s in range(a.steps):
-            served, dt = served_pass(server, reqs)
+            reqs = slices[s % a.slices]
+            got, dt = served_pass(server, reqs)
             times.append(dt)
-            log(f'served step {s + 1}/{a.steps}: {dt * 1e3:.1f} ms')
+            ncells += len(reqs)
+            res_timed += [r.result for r in got]
+            served.update({rq: r.result for rq, r in zip(reqs, got)})
+            log(f'served step {s + 1}/{a.steps} (slice {s % a.slices}): {len(reqs)} cells, {dt * 1e3:.1f} ms')
+        timed_wall = time.perf_counter() - t_timed
+        tot = [P.kernel_totals() for P in probs]
         step_s = float(np.mean(times))
-        served_rate = total_cells / step_s
-        # ---- open-loop latency at a fraction of the served rate
+        served_rate = ncells / float(np.sum(times))
+        # ---- open-loop latency at a fraction of the served rate, over the whole set
+        everything = [rq for sl in slices for rq in sl]
         rate = max(1.0, a.load * served_rate)
-        lat, lat_wall = poisson_latency(server, reqs, rate, a.latency_requests, a.seed)
+        lat, lat_wall = poisson_latency(server, everything, rate, a.latency_requests, a.seed)
         log(f'latency: {len(lat)} Poisson requests at {rate:.1f}/s in {lat_wall:.1f} s')
     finally:
         server.close()
 
-    # ---- batched: one persistent launch per class (kernel timing for the roofline)
-    kern_ms, kern_units, batched_s = [], [], 0.0
-    for P, cl in zip(probs, cells):
+    # ---- batched: one persistent launch per class on slice 0
+    batched_s, nbatched = 0.0, 0
+    for ci, P in enumerate(probs):
+        cl = [c for cj, c in slices[0] if cj == ci]
         t0 = time.perf_counter()
         P.search(cl, a.top_k, a.ab_budget, batched=True, **search)
         batched_s += time.perf_counter() - t0
-        ms, units = P.last_kernel(0)
-        kern_ms.append(ms)
-        kern_units.append(units)
-    log(f'batched: {len(probs)} launches, {batched_s * 1e3:.1f} ms')
+        nbatched += len(cl)
+    log(f'batched: {len(probs)} launches, {nbatched} cells, {batched_s * 1e3:.1f} ms')
 
     # ---- plain: one cell per call
     plain_lat, mismatch, nplain = [], 0, 0
-    for ci, (P, cl) in enumerate(zip(probs, cells)):
-        for j, c in enumerate(cl[:a.plain_cells]):
+    for ci, P in enumerate(probs):
+        for c in [c for cj, c in slices[0] if cj == ci][:a.plain_cells]:
             t0 = time.perf_counter()
             r = P.search([c], a.top_k, a.ab_budget, batched=False, **search)[0]
             plain_lat.append(time.perf_counter() - t0)
             nplain += 1
-            if key(r) != key(served[reqs.index((ci, c))].result):
+            if key(r) != key(served[(ci, c)]):
                 mismatch += 1
     plain_rate = nplain / sum(plain_lat) if plain_lat else 0.0
     log(f'plain: {nplain} cells, {plain_rate:.2f} cells/s, mismatches vs served {mismatch}')
 
-    res = [r.result for r in served]
-    streamed = sum(r.stream.cand for r in res)
-    completed = sum(r.completed for r in res)
-    dfs_nodes = sum(r.stream.dfs_nodes for r in res)
-    ab_nodes = sum(r.ab_nodes for r in res)
-    hits = sum(r.hit for r in res)
+    # ---- the leaf test against FP64 on a verify sample (slice 1): every certain decision
+    verify = verify_sample(a, probs, slices[1 % a.slices])
+    log(f'verify: {verify}')
 
-    # roofline of the dominant kernel (the stream kernel), from the batched launches
-    avg_ms = float(np.mean(kern_ms))
-    units_per_launch = float(np.mean(kern_units))
-    achieved = OPS_PER_CD_NODE * units_per_launch / (avg_ms / 1e3) / 1e12 if avg_ms > 0 else 0.0
-    traffic = None
-    tf = ROOT / 'profiles' / 'stream_kernel_traffic.json'
-    if tf.exists():
-        try:
-            traffic = json.loads(tf.read_text()).get('bytes_per_launch')
-        except ValueError:
-            traffic = None
-    roofline = {'bound': 'valu', 'achieved': round(achieved, 3), 'peak': round(INT_VALU_PEAK_TOPS, 2),
-                'unit': 'TOP/s (INT32 VALU)', 'frac': round(achieved / INT_VALU_PEAK_TOPS, 4), 'traffic': traffic,
-                'kernel': 'wzfh::stream_kernel',
-                'per_unit': f'{OPS_PER_CD_NODE} INT32 ops per counted C,D DFS node (rec() entry)',
-                'traffic_source': 'profiles/stream_kernel_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, '
-                                  'calibrated on a 256 MiB elementwise kernel)' if traffic else None,
-                'hbm_gb_per_s': round(traffic / (avg_ms / 1e3) / 1e9, 2) if traffic and avg_ms > 0 else None,
-                'hbm_frac': round(traffic / (avg_ms / 1e3) / HBM_PEAK_BPS, 5) if traffic and avg_ms > 0 else None,
-                'units_per_launch': int(units_per_launch), 'avg_launch_ms': round(avg_ms, 3)}
+    streamed = sum(r.stream.cand for r in res_timed)
+    completed = sum(r.completed for r in res_timed)
+    dfs_nodes = sum(r.stream.dfs_nodes for r in res_timed)
+    ab_nodes = sum(r.ab_nodes for r in res_timed)
+    hits = sum(r.hit for r in res_timed)
 
-    cpu = None if a.no_cpu else cpu_baseline(a, cls, probs, cells)
+    traffic = load_traffic()
+    sums = [[sum(t[k][j] for t in tot) for j in range(3)] for k in range(2)]
+    roof = {
+        'stream': kernel_roofline('wzfh::stream_kernel', OPS_PER_CD_NODE,
+                                  f'{OPS_PER_CD_NODE} INT32 ops per counted C,D DFS node (rec() entry)',
+                                  *sums[0], timed_wall, traffic.get('wzfh::stream_kernel')),
+        'completer': kernel_roofline('wzfh::complete_kernel', ops_per_ab_node(a.n),
+                                     f'{ops_per_ab_node(a.n)} INT32 ops (6L+n) per counted completer node',
+                                     *sums[1], timed_wall, traffic.get('wzfh::complete_kernel')),
+    }
+    dominant = max(roof, key=lambda k: roof[k]['avg_launch_ms'] * roof[k]['launches_timed'])
+
+    cpu = None if a.no_cpu else cpu_baseline(a, cls, probs, slices[0], served)
 
     line = {
         'metric': 'cells/sec & C,D cands/sec at n=44 (served vs plain GPU); p50/p99 cell latency',
         'value': round(served_rate, 3), 'unit': 'cells/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
         'ms_per_step': round(1e3 * step_s, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'int32 (DFS) + fp16x2/fp32 MFMA with fp64 recheck (leaf test)',
+        'dtype': 'int32 (DFS, completer) + fp16x2/fp32 MFMA with fp64 recheck (leaf test)',
         'data': 'synthetic: seeded splitmix64 sample of the real n=44 cell lists',
-        'config': {'workload': f'BS(45,44) front-only cell search: {len(cls)} classes x {a.cells_per_class} '
-                               f'seeded cells, node budget {a.node_budget}, cand cap {a.cand_cap}, '
-                               f'top-{a.top_k}, A,B budget {a.ab_budget}, served',
-                   'n': a.n, 'classes': len(cls), 'cells_per_step': total_cells, 'node_budget': a.node_budget,
+        'config': {'workload': f'BS(45,44): {len(cls)} classes x {a.cells_per_class} seeded cells, A,B budget '
+                               f'{a.ab_budget}, top-{a.top_k}; per-cell DFS-node budget {a.node_budget} (a policy of '
+                               f'this build: n=44 cells are ~1e11 nodes); a step = 1/{a.slices} of the set '
+                               f'(cells j = step mod {a.slices} of every class), served',
+                   'n': a.n, 'classes': len(cls), 'cells_per_class': a.cells_per_class, 'slices': a.slices,
+                   'cells_per_step': round(ncells / max(1, a.steps), 1), 'node_budget': a.node_budget,
                    'cand_cap': a.cand_cap, 'top_k': a.top_k, 'ab_budget': a.ab_budget, 'seed': a.seed,
-                   'parallelism': 'single GPU: continuous batching per class, persistent work-donating launches'},
+                   'parallelism': 'single GPU: continuous batching, persistent work-donating launches'},
         'served_cells_per_s': round(served_rate, 3),
         'plain_cells_per_s': round(plain_rate, 3),
         'served_over_plain': round(served_rate / plain_rate, 2) if plain_rate else None,
-        'batched_cells_per_s': round(total_cells / batched_s, 3) if batched_s else None,
-        'cands_streamed_per_s': round(streamed / step_s, 1),
-        'cands_completed_per_s': round(completed / step_s, 1),
-        'dfs_nodes_per_s': round(dfs_nodes / step_s, 1),
+        'batched_cells_per_s': round(nbatched / batched_s, 3) if batched_s else None,
+        'cands_streamed_per_s': round(streamed / float(np.sum(times)), 1),
+        'cands_completed_per_s': round(completed / float(np.sum(times)), 1),
+        'dfs_nodes_per_s': round(dfs_nodes / float(np.sum(times)), 1),
+        'ab_nodes_per_s': round(ab_nodes / float(np.sum(times)), 1),
         'served_latency_ms': {'p50': pct(lat, 50), 'p99': pct(lat, 99), 'offered_per_s': round(rate, 2),
-                              'requests': len(lat)},
+                              'load': a.load, 'requests': len(lat)},
         'plain_latency_ms': {'p50': pct(plain_lat, 50), 'p99': pct(plain_lat, 99), 'cells': nplain},
-        'hits': hits, 'dfs_nodes_per_step': dfs_nodes, 'ab_nodes_per_step': ab_nodes,
-        'plain_vs_served_mismatch': mismatch, 'setup_s': round(setup_s, 1),
-        'roofline': roofline, 'cpu_baseline': cpu, 'autotune': tune,
+        'hits': hits, 'plain_vs_served_mismatch': mismatch, 'false_rejects': verify.get('false_decisions'),
+        'verify': verify, 'setup_s': round(setup_s, 1), 'timed_wall_s': round(timed_wall, 3),
+        'roofline': dict(roof[dominant], dominant=True), 'rooflines': roof,
+        'cpu_baseline': cpu, 'autotune': tune,
     }
     print(json.dumps(line), flush=True)
     for P in probs:
         P.close()
 
 
-def probe_traffic(a):
-    """For `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: first an elementwise kernel that
-    reads and writes exactly 256 MiB each (the counters' calibration), then one stream launch of
-    the bench workload (the first class, --cells-per-class seeded cells, the node budget) with the
-    bench's tuned launch settings when profiles/stream_kernel_traffic.json names them."""
-    import torch
-    x = torch.zeros(64 << 20, dtype=torch.int32, device='cuda')
-    y = torch.empty_like(x)
-    torch.add(x, 1, out=y)
-    torch.cuda.synchronize()
-    P = wz.Problem(a.n, classes(a.n)[0], canon=1, order=1)
-    cells = sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000)
-    tf = ROOT / 'profiles' / 'stream_kernel_traffic.json'
-    if tf.exists():
-        t = json.loads(tf.read_text()).get('tuning')
-        if t:
-            P.set_tuning(wz.Tuning(*[t.get(k, d) for k, d in (('inner_moves', 16), ('donate_hops', 1),
-                                                              ('static_window', 4), ('complete_split', 3),
-                                                              ('idle_sleep', 64), ('blocks_per_cu', 4))]))
-    st, _, _ = P.stream(cells, node_budget=a.node_budget, want_cands=False)
-    ms, units = P.last_kernel(0)
-    print(json.dumps({'calibration_bytes_each_way': 256 << 20, 'cells': len(cells), 'stream_ms': round(ms, 3),
-                      'counted_nodes': units}), flush=True)
-    P.close()
+def serve_sweep(a):
+    """Serving configuration: every signature class of every n in 24..45 is a device-resident
+    problem; a request draws n uniformly, then a class of that n uniformly, then a cell of the
+    class's live list (seeded), and searches it with --node-budget DFS nodes, top --top-k and
+    A,B budget --sweep-ab-budget. First the closed-loop capacity (all requests at once), then
+    open-loop Poisson streams at fractions of it up to past saturation: achieved cells/s and
+    p50 / p99 latency per offered rate; the plain path (one cell per launch, one at a time) on a
+    sample beside it. Served results are compared with the same cells searched alone."""
+    rng = random.Random(a.seed * 31 + 5)
+    t0 = time.time()
+    probs, index = [], {}
+    for n in range(24, 46):
+        for sig in classes(n):
+            index[(n, sig)] = len(probs)
+            probs.append(wz.Problem(n, sig, canon=1, order=1, m6=1, thm211b=1, thm212=1))
+        log(f'sweep setup n={n}: {len(probs)} problems, {time.time() - t0:.0f} s')
+    live = [P.live_cells() for P in probs]
+    pool = []
+    for _ in range(4096):
+        n = rng.randrange(24, 46)
+        sig = rng.choice(classes(n))
+        pi = index[(n, sig)]
+        pool.append((pi, live[pi][rng.randrange(len(live[pi]))]))
+    setup_s = time.time() - t0
+    search = dict(node_budget=a.node_budget, cand_cap=a.cand_cap)
+    server = wz.Server(probs, a.top_k, a.sweep_ab_budget, max_batch=a.max_batch, **search)
+    rows, served = [], {}
+    try:
+        got, dt = served_pass(server, pool[:a.sweep_requests])   # warm-up and closed-loop capacity
+        got, dt = served_pass(server, pool[:a.sweep_requests])
+        capacity = a.sweep_requests / dt
+        served.update({rq: r.result for rq, r in zip(pool, got)})
+        log(f'closed-loop capacity: {capacity:.1f} cells/s ({a.sweep_requests} requests in {dt:.2f} s)')
+        for frac in (0.2, 0.4, 0.6, 0.8, 1.0, 1.25):
+            rate = frac * capacity
+            lat, wall = poisson_latency(server, pool, rate, a.sweep_requests, a.seed + int(100 * frac))
+            rows.append({'offered_frac': frac, 'offered_per_s': round(rate, 2),
+                         'achieved_per_s': round(len(lat) / wall, 2), 'p50_ms': pct(lat, 50), 'p99_ms': pct(lat, 99)})
+            log(f'offered {rate:.1f}/s: {rows[-1]}')
+    finally:
+        server.close()
+    plain_lat, mism = [], 0
+    for pi, c in pool[:48]:
+        t1 = time.perf_counter()
+        r = probs[pi].search([c], a.top_k, a.sweep_ab_budget, batched=False, **search)[0]
+        plain_lat.append(time.perf_counter() - t1)
+        if (pi, c) in served and key(r) != key(served[(pi, c)]):
+            mism += 1
+    plain_rate = len(plain_lat) / sum(plain_lat)
+    line = {'metric': 'served cells/s and p50/p99 latency vs offered rate, n ~ U{24..45}',
+            'value': round(capacity, 3), 'unit': 'cells/s', 'n_gpus': 1,
+            'config': {'workload': f'serving: n ~ U{{24..45}}, class ~ U(classes(n)), seeded cells; node budget '
+                                   f'{a.node_budget} per cell, top-{a.top_k}, A,B budget {a.sweep_ab_budget}',
+                       'problems': len(probs), 'requests_per_rate': a.sweep_requests},
+            'closed_loop_cells_per_s': round(capacity, 3), 'plain_cells_per_s': round(plain_rate, 3),
+            'served_over_plain': round(capacity / plain_rate, 2),
+            'plain_latency_ms': {'p50': pct(plain_lat, 50), 'p99': pct(plain_lat, 99), 'cells': len(plain_lat)},
+            'sweep': rows, 'plain_vs_served_mismatch': mism, 'setup_s': round(setup_s, 1)}
+    print(json.dumps(line), flush=True)
+    for P in probs:
+        P.close()
 
 
-def cpu_baseline(a, cls, probs, cells):
-    """The oracle port (oracle/wz_oracle.cpp) on the host cores: the same cell unit on a
-    bounded sample (--cpu-cells cells of each class, evenly spaced in its seeded sample), one cell
-    per thread."""
+def verify_sample(a, probs, reqs):
+    """Streams --verify-cells cells per class of one slice with WZFH_HALL_VERIFY=1: the FP64 test
+    also runs on every leaf and every certain matrix-core decision is compared with it."""
+    keep = {k: os.environ.get(k) for k in ('WZFH_HALL_VERIFY', 'WZFH_TELEMETRY')}
+    os.environ['WZFH_HALL_VERIFY'] = '1'
+    os.environ['WZFH_TELEMETRY'] = '1'
+    leaves = uncertain = bad = cells = 0
+    try:
+        for ci, P in enumerate(probs):
+            cl = [c for cj, c in reqs if cj == ci][:a.verify_cells]
+            P.stream(cl, node_budget=a.node_budget, want_cands=False)
+            leaves += P.last_kernel(5)[1]
+            uncertain += P.last_kernel(6)[1]
+            bad += P.last_kernel(7)[1]
+            cells += len(cl)
+    finally:
+        for k, v in keep.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return {'cells': cells, 'leaves_tested': leaves, 'decided_in_fp64': uncertain, 'false_decisions': bad,
+            'margin': round(probs[0].last_kernel(10)[0], 4)}
+
+
+def cpu_cores():
+    """(threads used, what the host reports). The box gives one GPU a 16-core share."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return affinity, {'nproc_affinity': affinity, 'os_cpu_count': os.cpu_count()}
+
+
+def cpu_baseline(a, cls, probs, reqs, served):
+    """The oracle port (oracle/wz_oracle.cpp, wzo_cell_search) on the host cores: the same cell
+    unit on a bounded sample of the same cells (every --cpu-stride-th cell of slice 0 per class),
+    one cell per thread, results compared with the served ones cell by cell."""
     sys.path.insert(0, str(ROOT / 'tests'))
     try:
         from oracle_lib import OracleProblem
     except Exception as e:  # oracle not built: no baseline
         return {'error': f'oracle unavailable: {e}'}
-    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    avail, host = cpu_cores()
+    threads = a.cpu_threads or min(16, avail)
     work, oprobs = [], []
     t_setup = time.time()
-    for P_gpu, sig, cl in zip(probs, cls, cells):
-        pick = cl[::max(1, len(cl) // max(1, a.cpu_cells))][:a.cpu_cells]   # spread over the seeded sample
-        P = OracleProblem(a.n, sig, canon=1, order=1, cells=[P_gpu.cell(pi)[:2] for pi in pick])
-        oprobs.append(P)
-        work += [(P, j) for j in range(len(pick))]
+    for ci, (P_gpu, sig) in enumerate(zip(probs, cls)):
+        pick = [c for cj, c in reqs if cj == ci][::a.cpu_stride]
+        O = OracleProblem(a.n, sig, canon=1, order=1, cells=[P_gpu.cell(pi)[:2] for pi in pick])
+        oprobs.append(O)
+        work += [(ci, O, j, c) for j, c in enumerate(pick)]
     setup = time.time() - t_setup
-    cap = a.cand_cap if a.cand_cap > 0 else (1 << 22)
 
     def one(item):
-        P, j = item
-        got, v, nd, idx, st = P.front_budget(j, cap, a.node_budget, a.top_k, a.ab_budget)
-        return got, st.dfs_nodes
+        ci, O, j, c = item
+        r, _ = O.search(j, a.cand_cap, a.node_budget, a.top_k, a.ab_budget)
+        return ci, c, r
 
     t0 = time.perf_counter()
-    cands = nodes = 0
+    cands = nodes = mism = 0
     with ThreadPoolExecutor(threads) as ex:
-        for k, (c, nd) in enumerate(ex.map(one, work)):
-            cands += c
-            nodes += nd
-            if (k + 1) % threads == 0:
+        for k, (ci, c, r) in enumerate(ex.map(one, work)):
+            cands += r.cand
+            nodes += r.dfs_nodes
+            g = served[(ci, c)]
+            want = (r.cand, r.dfs_nodes, r.completed, r.hit, r.hit_rank, r.ab_nodes, r.clean_no, r.aborted)
+            got = (g.stream.cand, g.stream.dfs_nodes, g.completed, g.hit, g.hit_rank, g.ab_nodes, g.clean_no,
+                   g.aborted)
+            if want != got or (r.hit and (list(wz.unpack(g.A, a.n + 1)) != list(r.A[:a.n + 1]) or
+                                          list(wz.unpack(g.B, a.n + 1)) != list(r.B[:a.n + 1]))):
+                mism += 1
+            if (k + 1) % 64 == 0:
                 log(f'cpu baseline: {k + 1}/{len(work)} cells')
     dt = time.perf_counter() - t0
+    model = _cpu_model()
     return {'value': round(len(work) / dt, 4), 'unit': 'cells/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{len(work)} cells ({a.cpu_cells} per class, evenly spaced in each class\'s seeded sample) '
-                      f'of the same workload and unit',
+            'sample': f'{len(work)} cells (every {a.cpu_stride}th cell of slice 0 of each class) of the same '
+                      f'workload and unit, one cell per thread',
+            'mismatches_vs_served': mism, 'cells_compared': len(work),
             'cands_per_s': round(cands / dt, 1), 'dfs_nodes_per_s': round(nodes / dt, 1), 'seconds': round(dt, 2),
-            'setup_s': round(setup, 1), 'cpu_model': _cpu_model()}
+            'setup_s': round(setup, 1), 'cpu_model': model, 'host': host,
+            'port_vs_reference': 'profiles/cpu_port_vs_reference.md'}
 
 
 def _cpu_model():
@@ -373,6 +492,63 @@ def _cpu_model():
     except OSError:
         pass
     return None
+
+
+def probe_traffic(a):
+    """For `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: first an elementwise kernel that
+    reads and writes exactly 256 MiB each (the counters' calibration), then one batched search
+    of the first class's cells of slice 0 (the batch a class launch sees in the served steps)
+    with the default launch settings."""
+    import torch
+    x = torch.zeros(64 << 20, dtype=torch.int32, device='cuda')
+    y = torch.empty_like(x)
+    torch.add(x, 1, out=y)
+    torch.cuda.synchronize()
+    P = wz.Problem(a.n, classes(a.n)[0], canon=1, order=1)
+    cells = sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000)[0::a.slices]
+    P.search(cells, a.top_k, a.ab_budget, cand_cap=a.cand_cap, node_budget=a.node_budget, batched=True)
+    tot = P.kernel_totals()
+    print(json.dumps({'calibration_bytes_each_way': 256 << 20, 'cells': len(cells),
+                      'stream': {'ms': tot[0][0], 'units': tot[0][1], 'launches': tot[0][2]},
+                      'completer': {'ms': tot[1][0], 'units': tot[1][1], 'launches': tot[1][2]}}), flush=True)
+    P.close()
+
+
+def traffic_summary(fetch_csv, write_csv):
+    """Per-kernel average FETCH_SIZE / WRITE_SIZE per dispatch from two rocprofv3 --pmc runs of
+    --probe-traffic, scaled by what the counters read for the calibration kernel."""
+    import csv
+
+    def per_kernel(path, counter):
+        acc = {}
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get('Counter_Name') != counter:
+                    continue
+                name = row['Kernel_Name']
+                acc.setdefault(name, []).append(float(row['Counter_Value']))
+        return acc
+
+    fetch, write = per_kernel(fetch_csv, 'FETCH_SIZE'), per_kernel(write_csv, 'WRITE_SIZE')
+    cal_f = [v for k, vs in fetch.items() if 'elementwise' in k.lower() or 'add' in k.lower() for v in vs]
+    cal_w = [v for k, vs in write.items() if 'elementwise' in k.lower() or 'add' in k.lower() for v in vs]
+    ff = (256 << 10) / max(cal_f) if cal_f else 1.0
+    fw = (256 << 10) / max(cal_w) if cal_w else 1.0
+    out = {'source': 'rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --probe-traffic, calibrated on a '
+                     '256 MiB elementwise kernel', 'fetch_factor': round(ff, 4), 'write_factor': round(fw, 4),
+           'kernels': {}}
+    for short in ('stream_kernel', 'complete_kernel', 'complete_reduce_kernel', 'stream_finalize'):
+        fk = [v for k, vs in fetch.items() if short in k for v in vs]
+        wk = [v for k, vs in write.items() if short in k for v in vs]
+        if not fk or not wk:
+            continue
+        fb, wb = 1024 * ff * float(np.mean(fk)), 1024 * fw * float(np.mean(wk))
+        out['kernels'][f'wzfh::{short}'] = {'dispatches': len(fk), 'fetch_kib_raw': round(float(np.mean(fk)), 1),
+                                            'write_kib_raw': round(float(np.mean(wk)), 1),
+                                            'bytes_per_launch': int(fb + wb), 'source': out['source']}
+    dst = ROOT / 'profiles' / 'kernel_traffic_n44.json'
+    dst.write_text(json.dumps(out, indent=1) + '\n')
+    print(json.dumps(out, indent=1))
 
 
 if __name__ == '__main__':

@@ -22,7 +22,7 @@ EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy',
            'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms',
            'wzfh_serve_open', 'wzfh_serve_submit', 'wzfh_serve_poll', 'wzfh_serve_close',
            'wzfh_join_open', 'wzfh_join_build', 'wzfh_join_probe', 'wzfh_join_close',
-           'wzfh_autotune', 'wzfh_get_tuning', 'wzfh_set_tuning')
+           'wzfh_autotune', 'wzfh_get_tuning', 'wzfh_set_tuning', 'wzfh_kernel_totals')
 
 
 class WzfhError(RuntimeError):
@@ -68,10 +68,16 @@ class Tuning(C.Structure):
     """wzfh_tuning: launch settings that change speed only."""
     _fields_ = [('inner_moves', C.c_int), ('donate_hops', C.c_int), ('static_window', C.c_int),
                 ('complete_split', C.c_int), ('idle_sleep', C.c_int), ('blocks_per_cu', C.c_int),
-                ('trials', C.c_int), ('reserved', C.c_int), ('stream_ms', C.c_double), ('complete_ms', C.c_double)]
+                ('trials', C.c_int), ('complete_cap_log2', C.c_int), ('stream_ms', C.c_double),
+                ('complete_ms', C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != 'reserved'}
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KernelTotals(C.Structure):
+    """wzfh_kernel_sums: per-handle running totals (index 0 stream kernel, 1 completer kernel)."""
+    _fields_ = [('ms', C.c_double * 2), ('units', C.c_longlong * 2), ('launches', C.c_longlong * 2)]
 
 
 class ServeOpts(C.Structure):
@@ -124,6 +130,7 @@ def lib():
         L.wzfh_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(SearchOpts), C.POINTER(Tuning)]
         L.wzfh_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.wzfh_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.wzfh_kernel_totals.argtypes = [vp, C.POINTER(KernelTotals), C.c_int]
         _lib = L
     return _lib
 
@@ -216,14 +223,14 @@ class Problem:
         return stats, offs, cd
 
     def complete(self, cd, budget=0, cell_of=None):
+        """A,B completions of packed (C, D) pairs (wzfh_complete)."""
         cd = np.ascontiguousarray(np.asarray(cd, dtype=np.uint64).reshape(-1, 2))
         nc = len(cd)
         out = (Completion * max(nc, 1))()
         co = None
         if cell_of is not None:
             co = np.ascontiguousarray(np.asarray(cell_of, dtype=np.int32))
-        rc = lib().wzfh_complete(self.h, cd.ctypes.data, co.ctypes.data if co is not None else None, nc,
-                                 budget, out)
+        rc = lib().wzfh_complete(self.h, cd.ctypes.data, co.ctypes.data if co is not None else None, nc, budget, out)
         got = [out[i] for i in range(nc)]
         if rc != OK:   # WZFH_EINTERNAL: verdict 4 somewhere (the reference's exit 5); results attached
             raise WzfhError(rc, lib().wzfh_last_error().decode(), got if rc == EINTERNAL else None)
@@ -256,6 +263,12 @@ class Problem:
 
     def set_tuning(self, t):
         _check(lib().wzfh_set_tuning(self.h, C.byref(t)))
+
+    def kernel_totals(self, reset=False):
+        """(ms, units, launches) per kernel (0 stream, 1 completer) since the last reset."""
+        t = KernelTotals()
+        _check(lib().wzfh_kernel_totals(self.h, C.byref(t), int(reset)))
+        return [(t.ms[k], t.units[k], t.launches[k]) for k in range(2)]
 
     def last_kernel(self, kind):
         ms, units = C.c_double(), C.c_longlong()

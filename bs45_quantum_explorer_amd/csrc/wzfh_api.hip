@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -29,6 +30,7 @@ size_t complete_lds_bytes(int L, int words);
 hipError_t launch_complete_reduce(const CompleteArgs &a, int nent, int *fv, long long *fn, uint64_t *fA,
                                   uint64_t *fB, int *fj, long long *fcc, long long *fcur, hipStream_t st);
 hipError_t complete_set_lds_limit(size_t bytes);
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 
 static thread_local std::string g_err;
 std::string set_error(const std::string &msg) { g_err = msg; return msg; }
@@ -63,9 +65,9 @@ struct DBuf {
 struct DeviceState {
   int device = 0, num_cu = 0;
   size_t complete_lds_max = 0;     // dynamic LDS bytes a completer block may use
-  wzfh_tuning tune{16, 1, 4, 4, 64, 2, 0, 0, 0.0, 0.0};   // launch settings (speed only)
+  wzfh_tuning tune{16, 1, 4, 4, 64, 2, 0, 14, 0.0, 0.0};   // launch settings (speed only)
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   // problem constants
   DBuf hcos, hsin, order, rej, outer2, outer3, rows, kab, hall_frag;
   StreamArgs base{};                       // tables + constants of the stream kernel
@@ -82,6 +84,11 @@ struct DeviceState {
        c_fv, c_fn, c_fA, c_fB, c_fj, c_fcc, c_fcur, c_scratch;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
+  // running totals since the last reset: device time of the stream kernel (kind 0) and of the
+  // completer kernel without its reduce (kind 1), their units and launch counts
+  std::mutex tot_mu;
+  double tot_ms[2] = {0, 0};
+  long long tot_units[2] = {0, 0}, tot_launches[2] = {0, 0};
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
   unsigned long long last_telem[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   double hall_eps = 0.0;           // margin of the matrix-core leaf test
@@ -337,7 +344,7 @@ static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncel
 }
 
 int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o,
-                      wzfh_cell_stats *stats, StreamOut &out) {
+                      wzfh_cell_stats *stats, StreamOut &out, const wzfh_problem *const *cell_prob) {
   DeviceState *S = P.dev;
   WZ_HIP(hipSetDevice(S->device));   // callers may be any host thread (serving workers)
   StreamArgs a = S->base;
@@ -357,7 +364,7 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
   std::vector<int> rootok(ncells);
   const int L = P.side_len(), m = P.m;
   for (int i = 0; i < ncells; i++) {
-    const HostCell &c = P.cells[cells[i]];
+    const HostCell &c = (cell_prob ? cell_prob[i] : &P)->cells[cells[i]];
     uint64_t rx = 0, ry = 0;
     bool ok = true;
     for (int q = 0; q < m; q++) {
@@ -483,6 +490,11 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     float ms = 0;
     WZ_HIP(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
     S->last_ms[0] = ms;
+    {
+      std::lock_guard<std::mutex> lk(S->tot_mu);
+      S->tot_ms[0] += ms;
+      S->tot_launches[0]++;
+    }
     if (a.telem) WZ_HIP(hipMemcpy(S->last_telem, S->telem.p, 80, hipMemcpyDeviceToHost));
     if (ctr[1] > cap) { cap = ctr[1] + (ctr[1] >> 2) + 1024; continue; }
     // ---- exact global order on the device; only the cell totals and kept candidates return
@@ -554,6 +566,10 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     }
     out.cell_begin[ncells] = (long long)out.cands.size();
     S->last_units[0] = units;
+    {
+      std::lock_guard<std::mutex> lk(S->tot_mu);
+      S->tot_units[0] += units;
+    }
     return WZFH_OK;
   }
   set_error("candidate buffer kept overflowing");
@@ -567,6 +583,9 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
 // is final, -1 marks an entry whose budget runs out inside item j (charges cc and counted
 // nodes nd before it, its own charges cj). ext = 0 runs every entry as one item whose budget
 // abort is counted exactly. Adds the kernels' time to *ms.
+// Result of one work item of a split launch (verdict 0..5, counted nodes, charges, A, B).
+struct ItemRes { int v; long long n, c; uint64_t A, B; };
+
 struct CompleteEntry {
   int cand;              // candidate index of the caller's arrays
   long long path;        // quad path above the entry's subtree (3 bits per depth)
@@ -575,11 +594,84 @@ struct CompleteEntry {
   long long pre_nodes;   // counted nodes of the candidate before the subtree
 };
 
+// Device buffers of one completer launch: per entry (inputs, reduce outputs), per work item
+// (results, charges) and the per-wave row scratch. They come either from the handle's growable
+// buffers or, for the allocation-free entry point, from a carve of this device's `scratch`.
+struct CompleteBufs {
+  uint64_t *cd = nullptr; int *roff = nullptr, *rlen = nullptr, *ebase = nullptr;
+  long long *epath = nullptr, *ebud = nullptr;
+  int *verdict = nullptr; long long *nodes = nullptr; uint64_t *A = nullptr, *B = nullptr;
+  long long *charges = nullptr, *icharged = nullptr;
+  uint64_t *row_scratch = nullptr; unsigned long long *ctr = nullptr;
+  int *fv = nullptr; long long *fn = nullptr; uint64_t *fA = nullptr, *fB = nullptr; int *fj = nullptr;
+  long long *fcc = nullptr, *fcur = nullptr;
+};
+
+// A bump carve of a fixed device range (256-byte aligned pieces); nullptr once it runs out.
+struct Carve {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  template <typename T> T *take(size_t bytes) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (at + bytes > cap) return nullptr;
+    used = at + bytes;
+    return (T *)(base + at);
+  }
+};
+
+// Per-item and per-entry bytes of a launch in a carve (with the alignment of each piece).
+constexpr size_t kItemBytes = 4 + 8 + 8 + 8 + 8 + 8;
+constexpr size_t kEntryBytes = 16 + 4 + 4 + 4 + 8 + 8 + 4 + 8 + 8 + 8 + 4 + 8 + 8;
+constexpr size_t kCarveSlack = 24 * 256 + 64;
+
+static int complete_buffers(DeviceState *S, Carve *carve, int nent, size_t nw, size_t row_scratch,
+                            CompleteBufs &b) {
+  if (!carve) {
+    if (S->c_cd.ensure((size_t)nent * 16) || S->c_roff.ensure(nent * 4) || S->c_rlen.ensure(nent * 4) ||
+        S->c_ebase.ensure(nent * 4) || S->c_epath.ensure(nent * 8) || S->c_ebud.ensure(nent * 8) ||
+        S->c_verdict.ensure(nw * 4) || S->c_nodes.ensure(nw * 8) || S->c_A.ensure(nw * 8) || S->c_B.ensure(nw * 8) ||
+        S->c_charges.ensure(nw * 8) || S->c_icharged.ensure(nw * 8) || S->c_scratch.ensure(row_scratch) ||
+        S->ctr.ensure(64) || S->c_fv.ensure(nent * 4) || S->c_fn.ensure(nent * 8) || S->c_fA.ensure(nent * 8) ||
+        S->c_fB.ensure(nent * 8) || S->c_fj.ensure(nent * 4) || S->c_fcc.ensure(nent * 8) ||
+        S->c_fcur.ensure(nent * 8))
+      return WZFH_ENOMEM;
+    b.cd = S->c_cd.as<uint64_t>(); b.roff = S->c_roff.as<int>(); b.rlen = S->c_rlen.as<int>();
+    b.ebase = S->c_ebase.as<int>(); b.epath = S->c_epath.as<long long>(); b.ebud = S->c_ebud.as<long long>();
+    b.verdict = S->c_verdict.as<int>(); b.nodes = S->c_nodes.as<long long>();
+    b.A = S->c_A.as<uint64_t>(); b.B = S->c_B.as<uint64_t>();
+    b.charges = S->c_charges.as<long long>(); b.icharged = S->c_icharged.as<long long>();
+    b.row_scratch = S->c_scratch.as<uint64_t>(); b.ctr = S->ctr.as<unsigned long long>();
+    b.fv = S->c_fv.as<int>(); b.fn = S->c_fn.as<long long>(); b.fA = S->c_fA.as<uint64_t>();
+    b.fB = S->c_fB.as<uint64_t>(); b.fj = S->c_fj.as<int>(); b.fcc = S->c_fcc.as<long long>();
+    b.fcur = S->c_fcur.as<long long>();
+    return WZFH_OK;
+  }
+  carve->used = 0;
+  b.ctr = carve->take<unsigned long long>(64);
+  b.cd = carve->take<uint64_t>((size_t)nent * 16); b.roff = carve->take<int>(nent * 4);
+  b.rlen = carve->take<int>(nent * 4); b.ebase = carve->take<int>(nent * 4);
+  b.epath = carve->take<long long>(nent * 8); b.ebud = carve->take<long long>(nent * 8);
+  b.verdict = carve->take<int>(nw * 4); b.nodes = carve->take<long long>(nw * 8);
+  b.A = carve->take<uint64_t>(nw * 8); b.B = carve->take<uint64_t>(nw * 8);
+  b.charges = carve->take<long long>(nw * 8); b.icharged = carve->take<long long>(nw * 8);
+  b.fv = carve->take<int>(nent * 4); b.fn = carve->take<long long>(nent * 8); b.fA = carve->take<uint64_t>(nent * 8);
+  b.fB = carve->take<uint64_t>(nent * 8); b.fj = carve->take<int>(nent * 4); b.fcc = carve->take<long long>(nent * 8);
+  b.fcur = carve->take<long long>(nent * 8);
+  b.row_scratch = carve->take<uint64_t>(row_scratch);
+  if (!b.ctr || !b.cd || !b.roff || !b.rlen || !b.ebase || !b.epath || !b.ebud || !b.verdict || !b.nodes || !b.A ||
+      !b.B || !b.charges || !b.icharged || !b.fv || !b.fn || !b.fA || !b.fB || !b.fj || !b.fcc || !b.fcur ||
+      !b.row_scratch) {
+    set_error("allocation-free completer: a launch does not fit the static scratch array");
+    return WZFH_ECAPACITY;
+  }
+  return WZFH_OK;
+}
+
 static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff, const int *rlen,
                            const std::vector<CompleteEntry> &ents, int maxrows, long long budget, int ext,
                            long long item_cap, int *v,
                            long long *nd, uint64_t *A, uint64_t *B, int *fj, long long *fcc, long long *fcur,
-                           double *ms) {
+                           double *ms, Carve *carve, std::vector<std::vector<ItemRes>> *items) {
   DeviceState *S = P.dev;
   const int nent = (int)ents.size();
   const int words = std::max(1, (maxrows + 63) / 64);
@@ -588,11 +680,17 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   for (const CompleteEntry &x : ents) root_pair = root_pair && x.base == 0;
   const int items_log2 = 3 * ext - (root_pair ? 2 : 0);
   const long long nwork = (long long)nent << items_log2;
-  const int blocks = (int)std::max(1ll, std::min((long long)S->num_cu * 8, nwork));
+  int blocks = (int)std::max(1ll, std::min((long long)S->num_cu * 8, nwork));
   const size_t lds_bytes = complete_lds_bytes(P.L1, words);
   const bool lds = lds_bytes <= S->complete_lds_max;
-  const size_t scratch = lds ? 64 : (size_t)blocks * lds_bytes;
   const size_t nw = (size_t)nwork;
+  if (carve && !lds) {   // row bitsets in global memory: as many waves as the carve holds
+    const size_t fixed = kCarveSlack + (size_t)nent * kEntryBytes + nw * kItemBytes;
+    const size_t room = carve->cap > fixed ? carve->cap - fixed : 0;
+    blocks = (int)std::min<size_t>((size_t)blocks, room / lds_bytes);
+    if (blocks < 1) { set_error("allocation-free completer: row lists too large for the static scratch array"); return WZFH_ECAPACITY; }
+  }
+  const size_t row_scratch = lds ? 64 : (size_t)blocks * lds_bytes;
   std::vector<uint64_t> ecd(2 * (size_t)nent);
   std::vector<int> ero(nent), erl(nent), ebase(nent);
   std::vector<long long> epath(nent), ebud(nent);
@@ -602,20 +700,18 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
     ero[e] = roff[x.cand]; erl[e] = rlen[x.cand];
     ebase[e] = x.base; epath[e] = x.path; ebud[e] = x.budget;
   }
-  if (S->c_cd.ensure((size_t)nent * 16) || S->c_roff.ensure(nent * 4) || S->c_rlen.ensure(nent * 4) ||
-      S->c_ebase.ensure(nent * 4) || S->c_epath.ensure(nent * 8) || S->c_ebud.ensure(nent * 8) ||
-      S->c_verdict.ensure(nw * 4) || S->c_nodes.ensure(nw * 8) || S->c_A.ensure(nw * 8) || S->c_B.ensure(nw * 8) ||
-      S->c_charges.ensure(nw * 8) || S->c_icharged.ensure(nw * 8) ||
-      S->c_scratch.ensure(scratch) || S->ctr.ensure(64))
-    return WZFH_ENOMEM;
-  WZ_HIP(hipMemcpyAsync(S->c_cd.p, ecd.data(), (size_t)nent * 16, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_roff.p, ero.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_rlen.p, erl.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_ebase.p, ebase.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_epath.p, epath.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemcpyAsync(S->c_ebud.p, ebud.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
-  WZ_HIP(hipMemsetAsync(S->ctr.p, 0, 64, S->stream));
-  if (ext > 0) WZ_HIP(hipMemsetAsync(S->c_icharged.p, 0, nw * 8, S->stream));
+  CompleteBufs bf;
+  if (int rc = complete_buffers(S, carve, nent, nw, row_scratch, bf)) return rc;
+  WZ_HIP(hipMemcpyAsync(bf.cd, ecd.data(), (size_t)nent * 16, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(bf.roff, ero.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(bf.rlen, erl.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(bf.ebase, ebase.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(bf.epath, epath.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
+  WZ_HIP(hipMemcpyAsync(bf.ebud, ebud.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
+  // counters start at zero (a fill kernel, not a memset node: callers of the allocation-free
+  // entry point may record this stream into a graph)
+  WZ_HIP(launch_zero(bf.ctr, 64, S->stream));
+  if (ext > 0) WZ_HIP(launch_zero(bf.icharged, nw * 8, S->stream));
   CompleteArgs a{};
   a.n = P.n; a.L = P.L1; a.m = P.m;
   a.absA = std::abs(P.cfg.a); a.absB = std::abs(P.cfg.b);
@@ -623,10 +719,10 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.ncand = nent; a.budget = budget;
   a.split_ext = ext; a.nwork = nwork; a.root_pair = root_pair ? 1 : 0; a.items_log2 = items_log2;
   a.item_cap = ext > 0 ? item_cap : 0;
-  a.ent_path = S->c_epath.as<long long>(); a.ent_base = S->c_ebase.as<int>(); a.ent_budget = S->c_ebud.as<long long>();
-  a.cd = S->c_cd.as<uint64_t>();
-  a.row_off = S->c_roff.as<int>(); a.row_len = S->c_rlen.as<int>();
-  a.rows = S->rows.p ? S->rows.as<uint64_t>() : S->c_cd.as<uint64_t>();
+  a.ent_path = bf.epath; a.ent_base = bf.ebase; a.ent_budget = bf.ebud;
+  a.cd = bf.cd;
+  a.row_off = bf.roff; a.row_len = bf.rlen;
+  a.rows = S->rows.p ? S->rows.as<uint64_t>() : bf.cd;
   for (int d = 0; d < 32; d++)
     for (int c = 0; c < 8; c++) {
       int r = c < P.m ? class_count(P.L1, c, P.m) : 0;
@@ -637,40 +733,41 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.outer2 = S->outer2.as<uint8_t>(); a.outer3 = S->outer3.as<uint8_t>();
   a.pos_pack = 0;
   for (int sb = 0; sb < 16; sb++) a.pos_pack |= (uint64_t)(kOuter.pos_index[sb] + 1) << (4 * sb);
-  a.scratch = S->c_scratch.as<uint64_t>(); a.scratch_words = words;
-  a.work_ctr = S->ctr.as<unsigned long long>();
-  a.verdict = S->c_verdict.as<int>(); a.nodes = S->c_nodes.as<long long>();
-  a.outA = S->c_A.as<uint64_t>(); a.outB = S->c_B.as<uint64_t>();
-  a.charges = S->c_charges.as<long long>();
-  a.item_charged = S->c_icharged.as<long long>();
+  a.scratch = bf.row_scratch; a.scratch_words = words;
+  a.work_ctr = bf.ctr;
+  a.verdict = bf.verdict; a.nodes = bf.nodes;
+  a.outA = bf.A; a.outB = bf.B;
+  a.charges = bf.charges;
+  a.item_charged = bf.icharged;
   WZ_HIP(hipEventRecord(S->ev[2], S->stream));
   WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, lds, lds ? lds_bytes : 0, S->stream));
+  WZ_HIP(hipEventRecord(S->ev[4], S->stream));
   // per-entry results: on the device for split launches (one wave per entry), from the
   // items themselves for unsplit ones; WZFH_CHECK_REDUCE=1 also scans the items on the host
   // and compares
-  if (ext > 0) {
-    if (S->c_fv.ensure(nent * 4) || S->c_fn.ensure(nent * 8) || S->c_fA.ensure(nent * 8) || S->c_fB.ensure(nent * 8) ||
-        S->c_fj.ensure(nent * 4) || S->c_fcc.ensure(nent * 8) || S->c_fcur.ensure(nent * 8))
-      return WZFH_ENOMEM;
-    WZ_HIP(launch_complete_reduce(a, nent, S->c_fv.as<int>(), S->c_fn.as<long long>(), S->c_fA.as<uint64_t>(),
-                                  S->c_fB.as<uint64_t>(), S->c_fj.as<int>(), S->c_fcc.as<long long>(),
-                                  S->c_fcur.as<long long>(), S->stream));
-  }
+  if (ext > 0)
+    WZ_HIP(launch_complete_reduce(a, nent, bf.fv, bf.fn, bf.fA, bf.fB, bf.fj, bf.fcc, bf.fcur, S->stream));
   WZ_HIP(hipEventRecord(S->ev[3], S->stream));
   const bool check = ext > 0 && getenv("WZFH_CHECK_REDUCE") && atoi(getenv("WZFH_CHECK_REDUCE"));
   std::vector<int> hv(nent), hj(nent);
   std::vector<long long> hn(nent), hcc(nent), hcj(nent);
   std::vector<uint64_t> hA(nent), hB(nent);
-  if (ext == 0 || check) {
-    std::vector<int> iv(nw);
-    std::vector<long long> in(nw), ic(nw);
-    std::vector<uint64_t> iA(nw), iB(nw);
+  std::vector<int> iv;
+  std::vector<long long> in, ic;
+  std::vector<uint64_t> iA, iB;
+  auto fetch_items = [&]() -> int {
+    if (!iv.empty()) return WZFH_OK;
+    iv.resize(nw); in.resize(nw); ic.resize(nw); iA.resize(nw); iB.resize(nw);
     WZ_HIP(hipMemcpyAsync(iv.data(), a.verdict, nw * 4, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipMemcpyAsync(in.data(), a.nodes, nw * 8, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipMemcpyAsync(ic.data(), a.charges, nw * 8, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipMemcpyAsync(iA.data(), a.outA, nw * 8, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipMemcpyAsync(iB.data(), a.outB, nw * 8, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipStreamSynchronize(S->stream));
+    return WZFH_OK;
+  };
+  if (ext == 0 || check) {
+    if (int rc = fetch_items()) return rc;
     // the same scan as the device reduce, on the host
     const long long np = 1ll << items_log2;
     for (int e = 0; e < nent; e++) {
@@ -688,6 +785,7 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
         cum_n += in[u];
         if (iv[u] == 0 || iv[u] == 3 || iv[u] == 4) {   // a hit, or (unsplit) an exactly counted abort
           vo = iv[u]; Ao = iA[u]; Bo = iB[u];
+          if (ext > 0) cum_c += ic[u];   // a split hit reports the charges through it
           break;
         }
         cum_c += ic[u];
@@ -696,13 +794,13 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
     }
   }
   if (ext > 0) {
-    WZ_HIP(hipMemcpyAsync(v, S->c_fv.p, nent * 4, hipMemcpyDeviceToHost, S->stream));
-    WZ_HIP(hipMemcpyAsync(nd, S->c_fn.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
-    WZ_HIP(hipMemcpyAsync(A, S->c_fA.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
-    WZ_HIP(hipMemcpyAsync(B, S->c_fB.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
-    WZ_HIP(hipMemcpyAsync(fj, S->c_fj.p, nent * 4, hipMemcpyDeviceToHost, S->stream));
-    WZ_HIP(hipMemcpyAsync(fcc, S->c_fcc.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
-    WZ_HIP(hipMemcpyAsync(fcur, S->c_fcur.p, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(v, bf.fv, nent * 4, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(nd, bf.fn, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(A, bf.fA, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(B, bf.fB, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(fj, bf.fj, nent * 4, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(fcc, bf.fcc, nent * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipMemcpyAsync(fcur, bf.fcur, nent * 8, hipMemcpyDeviceToHost, S->stream));
     WZ_HIP(hipStreamSynchronize(S->stream));
     if (check)
       for (int e = 0; e < nent; e++) {
@@ -718,15 +816,35 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
       v[e] = hv[e]; nd[e] = hn[e]; A[e] = hA[e]; B[e] = hB[e]; fj[e] = hj[e]; fcc[e] = hcc[e]; fcur[e] = hcj[e];
     }
   }
+  if (items) {   // entries whose scan met a capped item first: their items, for the host
+    items->assign(nent, {});
+    const long long np = 1ll << items_log2;
+    for (int e = 0; e < nent; e++)
+      if (v[e] == -2) {
+        if (int rc = fetch_items()) return rc;
+        std::vector<ItemRes> &dst = (*items)[e];
+        dst.resize(np);
+        for (long long p = 0; p < np; p++) {
+          const size_t u = (size_t)((long long)e * np + p);
+          dst[p] = ItemRes{iv[u], in[u], ic[u], iA[u], iB[u]};
+        }
+      }
+  }
   if (root_pair)   // item index -> quad path digits below the entry
     for (int e = 0; e < nent; e++)
       if (fj[e] >= 0) {
         const int lo = 3 * (ext - 1);
         fj[e] = ((6 | (fj[e] >> lo)) << lo) | (fj[e] & ((1 << lo) - 1));
       }
-  float t = 0;
+  float t = 0, tk = 0;
   WZ_HIP(hipEventElapsedTime(&t, S->ev[2], S->ev[3]));
+  WZ_HIP(hipEventElapsedTime(&tk, S->ev[2], S->ev[4]));
   *ms += t;
+  {
+    std::lock_guard<std::mutex> lk(S->tot_mu);
+    S->tot_ms[1] += tk;
+    S->tot_launches[1]++;
+  }
   return WZFH_OK;
 }
 
@@ -740,13 +858,13 @@ static int complete_split_ext(const wzfh_problem &P) {
   if (ext >= P.L1 / 2) ext = 0;
   return ext;
 }
-// an abort item this small (charges) is walked as one item; WZFH_COMPLETE_EXACT_ITEM overrides
-// WZFH_COMPLETE_ITEM_CAP = C > 0: first-round split items stop after C charges and an entry
-// with such an item before its decision is split deeper instead. Off by default: it halves a
-// lone heavy candidate's time but the deeper rounds cost batches more than they save.
-static long long item_charge_cap() {
-  const char *e = getenv("WZFH_COMPLETE_ITEM_CAP");
-  return e ? std::max(0ll, atoll(e)) : 0;
+// Split items stop after 2^complete_cap_log2 charges (the tuning; WZFH_COMPLETE_ITEM_CAP = C
+// overrides with C charges, 0 = no cap); the subtree of a capped item is split again in the next
+// round, so no item walks much more than the cap.
+static long long item_charge_cap(const wzfh_problem &P) {
+  if (const char *e = getenv("WZFH_COMPLETE_ITEM_CAP")) return std::max(0ll, atoll(e));
+  const int lg = P.dev->tune.complete_cap_log2;
+  return lg > 0 ? 1ll << std::min(lg, 40) : 0;
 }
 
 static long long exact_item_charges() {
@@ -754,21 +872,60 @@ static long long exact_item_charges() {
   return e ? std::max(0ll, atoll(e)) : 2048;   // a whole walk of 2048 charges is a few ms on one wave
 }
 
-// Cells of one search batch that fit in a 1/(2 nshare) share of the device's free memory:
-// per cell the stream's work records (16 K donated records of 68 B, the static ones) and
-// candidates, and the completer's split items (top_k candidates x items per candidate x 44 B).
+// Cells of one search batch that fit in a 1/(2 nshare) share of the device's free memory (read
+// at every call): per cell the stream's work records (16 K donated records of 68 B, the static
+// ones), its candidate records (32 B raw + 32 B kept, up to the cap plus a margin, or 1,024
+// without a cap), and the completer's split items (top_k candidates x items x 44 B).
 long long serve_cell_slots(wzfh_problem &P, int nshare, const wzfh_search_opts &o) {
   size_t free_b = 0, total_b = 0;
   if (!P.dev || hipSetDevice(P.dev->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
     return 1ll << 30;
   const int ext = complete_split_ext(P);
   const double items = ext > 0 ? (double)(1ll << (3 * ext)) : 1.0;
-  const double per_cell = 16384.0 * 72.0 + (double)std::max(1, o.top_k) * items * 44.0 + 65536.0;
+  const double cands = o.stream.cand_cap > 0 ? (double)o.stream.cand_cap + 512.0 : 1024.0;
+  const double per_cell = 16384.0 * 72.0 + cands * 64.0 + (double)std::max(1, o.top_k) * items * 44.0 + 65536.0;
   return std::max(1ll, (long long)((double)free_b / (2.0 * std::max(1, nshare)) / per_cell));
 }
 
-static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
-                        wzfh_completion *out) {
+// Host side of the completer: rounds of launches over jobs until every candidate is decided.
+//
+// A job is a subtree of one candidate (the whole tree at first) with the charges left to the
+// candidate's budget at its start. A split launch runs a job's quad paths `ext` levels further
+// down as work items, every item stopping after `cap` charges; the device reduce scans the items
+// in path order (the reference's DFS order) and reports the decision, or the item in which the
+// budget runs out (that item's subtree becomes an exact-abort job), or that a capped item comes
+// before any decision. In that last case the job becomes a frame: its items' results come to
+// the host, every capped item's subtree becomes a job of the next round (all launched together,
+// each with the charges left as if the capped items before it had charged only what they did
+// before stopping -- an upper bound), and the frame is scanned in path order as its capped items
+// resolve. A capped item's result is used as is when its decision cannot depend on the budget;
+// when the budget runs out inside it, its subtree is run once more with the exact charges left.
+// So every heavy subtree is split again and again until its pieces are small, and the whole
+// candidate's verdict, node count and first A,B are the reference's.
+namespace {
+struct Result { int v; long long n, c, bud; uint64_t A, B; bool done; };
+struct Job {
+  int cand; long long path; int base;
+  long long budget;          // charges left at the subtree's start
+  long long pre_n, pre_c;    // nodes / charges of the target before this job
+  int owner, slot;           // target: frame `owner`'s capped-item slot, or the candidate (-1)
+  bool whole;                // walk as one item (an exactly counted abort)
+};
+struct Frame {
+  int cand; long long path; int base, ext; bool root_pair;
+  long long budget, pre_n, pre_c;
+  int owner, slot;
+  std::vector<ItemRes> items;
+  std::vector<int> slot_of;  // item -> its capped-item slot (-1: decided by the launch)
+  std::vector<Result> subs;
+  size_t next = 0;
+  long long cum_n = 0, cum_c = 0;
+  bool closed = false;
+};
+}  // namespace
+
+int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                 wzfh_completion *out, Carve *carve) {
   DeviceState *S = P.dev;
   if (ncand <= 0) return WZFH_OK;
   WZ_HIP(hipSetDevice(S->device));
@@ -785,74 +942,221 @@ static int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of,
     rlen[i] = P.row_len[li];
     maxrows = std::max(maxrows, rlen[i]);
   }
-  std::vector<int> fv(ncand, 2);
-  std::vector<long long> fnd(ncand, 0);
-  std::vector<uint64_t> fA(ncand, 0), fB(ncand, 0);
+  const bool limited = budget > 0;
+  std::vector<Result> fin(ncand, Result{2, 0, 0, 0, 0, 0, false});
   double ms = 0;
   const int ext0 = complete_split_ext(P);
   const int max_base = std::min(P.L1 / 2 - 1, 21);   // paths are 3 bits per depth in 64 bits
   const long long exact_item = exact_item_charges();
-  std::vector<CompleteEntry> pending(ncand);
-  for (int i = 0; i < ncand; i++) pending[i] = CompleteEntry{i, 0, 0, budget, 0};
-  // entries to split further, and entries walked as one item (exact abort count)
-  // entries split ext0 levels with a per-item charge cap, entries split deeper without a cap
-  // (an item hit the cap before the entry's decision), and entries walked as one item (exact
-  // abort count)
-  const int ext_deep = std::min(6, std::min(max_base, ext0 + 2));
-  const long long cap = item_charge_cap();
-  std::vector<CompleteEntry> split = ext0 > 0 ? pending : std::vector<CompleteEntry>{};
-  std::vector<CompleteEntry> deep, whole = ext0 > 0 ? std::vector<CompleteEntry>{} : pending;
-  while (!split.empty() || !deep.empty() || !whole.empty()) {
-    std::vector<CompleteEntry> next_split, next_deep, next_whole;
-    for (int pass = 0; pass < 3; pass++) {
-      std::vector<CompleteEntry> &ents = pass == 0 ? split : pass == 1 ? deep : whole;
-      const int ext = pass == 0 ? ext0 : pass == 1 ? ext_deep : 0;
-      const long long icap = pass == 0 && ext_deep > ext0 ? cap : 0;
-      const int chunk = std::max(1, (1 << 22) >> (3 * ext));   // at most 4M items per launch
-      for (size_t c0 = 0; c0 < ents.size(); c0 += chunk) {
-        const std::vector<CompleteEntry> part(ents.begin() + c0, ents.begin() + std::min(ents.size(), c0 + chunk));
+  const long long cap = ext0 > 0 ? item_charge_cap(P) : 0;
+  std::vector<Frame> frames;
+  std::vector<Job> jobs, next_jobs;
+  for (int i = 0; i < ncand; i++) jobs.push_back(Job{i, 0, 0, budget, 0, 0, -1, i, ext0 == 0});
+
+  std::function<void(int, int, Result)> deliver;
+  // scans frame f from its next item; returns when it waits for a capped item or is closed
+  auto resume = [&](int f) {
+    Frame &F = frames[f];
+    auto finish = [&](Result r) {
+      F.closed = true;
+      r.n += F.pre_n; r.c += F.pre_c; r.bud = F.pre_c + F.budget;
+      deliver(F.owner, F.slot, r);
+    };
+    auto abort_in = [&](size_t i, bool small) {   // the budget runs out inside item i: exact rerun
+      const int lo = 3 * (F.ext - 1);
+      const long long dig = F.root_pair ? (((6 | ((long long)i >> lo)) << lo) | ((long long)i & ((1ll << lo) - 1)))
+                                        : (long long)i;
+      Job y{F.cand, (F.path << (3 * F.ext)) | dig, F.base + F.ext, F.budget - F.cum_c, F.pre_n + F.cum_n,
+            F.pre_c + F.cum_c, F.owner, F.slot, false};
+      y.whole = small || y.base + ext0 > max_base;
+      F.closed = true;
+      next_jobs.push_back(y);
+    };
+    while (!F.closed && F.next < F.items.size()) {
+      const size_t i = F.next;
+      const ItemRes &it = F.items[i];
+      if (it.v == 1) { finish(Result{1, 0, 0, 0, 0, 0, true}); return; }   // pre-rejected candidate
+      if (it.v == 5) {
+        // a capped item launched no subtree job only when the charges before it plus its own
+        // partial charges already pass the budget: the budget runs out inside it
+        if (F.slot_of[i] < 0) { abort_in(i, false); return; }
+        const Result &r = F.subs[F.slot_of[i]];
+        if (!r.done) return;   // waits for the capped item's subtree
+        const long long left = F.budget - F.cum_c;
+        if (limited && (r.v == 3 || r.c > left)) {
+          if (r.v == 3 && r.bud == left) { finish(Result{3, F.cum_n + r.n, F.cum_c + r.c, 0, 0, 0, true}); return; }
+          abort_in(i, false);
+          return;
+        }
+        if (r.v == 0 || r.v == 4) { finish(Result{r.v, F.cum_n + r.n, F.cum_c + r.c, 0, r.A, r.B, true}); return; }
+        F.cum_n += r.n; F.cum_c += r.c; F.next++;
+        continue;
+      }
+      if (it.v == 3 || (limited && F.cum_c + it.c > F.budget)) { abort_in(i, it.c <= exact_item); return; }
+      if (it.v == 0 || it.v == 4) { finish(Result{it.v, F.cum_n + it.n, F.cum_c + it.c, 0, it.A, it.B, true}); return; }
+      F.cum_n += it.n; F.cum_c += it.c; F.next++;
+    }
+    if (!F.closed) finish(Result{2, F.cum_n, F.cum_c, 0, 0, 0, true});
+  };
+  deliver = [&](int owner, int slot, Result r) {
+    r.done = true;
+    if (owner < 0) { fin[slot] = r; return; }
+    frames[owner].subs[slot] = r;
+    resume(owner);
+  };
+
+  int round = 0;
+  while (!jobs.empty()) {
+    if (++round > 4096) { set_error("completer: rounds did not converge (internal error)"); return WZFH_EKERNEL; }
+    next_jobs.clear();
+    // launches: exact walks, first-round splits (root canon pair), deeper splits
+    for (int kind = 0; kind < 3; kind++) {
+      std::vector<Job> group;
+      for (const Job &x : jobs)
+        if ((kind == 0) == x.whole && (kind == 0 || (kind == 1) == (x.base == 0))) group.push_back(x);
+      if (group.empty()) continue;
+      const int ext = kind == 0 ? 0 : ext0;
+      const long long icap = kind == 0 ? 0 : cap;
+      // at most 4M items per launch (what the static scratch holds for the allocation-free path)
+      long long max_items = 1ll << 22;
+      if (carve) max_items = std::min<long long>(max_items, (long long)((carve->cap - kCarveSlack) / (kItemBytes + kEntryBytes)));
+      const int chunk = (int)std::max(1ll, max_items >> (3 * ext));
+      for (size_t c0 = 0; c0 < group.size(); c0 += chunk) {
+        std::vector<CompleteEntry> part;
+        for (size_t e = c0; e < std::min(group.size(), c0 + chunk); e++)
+          part.push_back(CompleteEntry{group[e].cand, group[e].path, group[e].base, group[e].budget, 0});
         const int ne = (int)part.size();
         std::vector<int> v(ne), j(ne);
         std::vector<long long> nd(ne), cc(ne), cj(ne);
         std::vector<uint64_t> A(ne), B(ne);
+        std::vector<std::vector<ItemRes>> items;
         int rc = complete_launch(P, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, icap, v.data(),
-                                 nd.data(), A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms);
+                                 nd.data(), A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms, carve, &items);
         if (rc != WZFH_OK) return rc;
+        const bool root_pair = ext > 0 && !P.cfg.no_canon_ab && kind == 1;
         for (int e = 0; e < ne; e++) {
-          const CompleteEntry &x = part[e];
+          const Job &x = group[c0 + e];
           if (v[e] >= 0) {   // decided: hit, clean, pre-rejected, or an exactly counted abort
-            fv[x.cand] = v[e];
-            fnd[x.cand] = x.pre_nodes + nd[e];
-            fA[x.cand] = A[e];
-            fB[x.cand] = B[e];
+            deliver(x.owner, x.slot, Result{v[e], x.pre_n + nd[e], x.pre_c + cc[e], x.pre_c + x.budget, A[e], B[e], true});
             continue;
           }
-          if (v[e] == -2) { next_deep.push_back(x); continue; }   // the same entry, split deeper
-          // the budget runs out inside item j: its subtree with the charges that remain
-          CompleteEntry y{x.cand, (x.path << (3 * ext)) | j[e], x.base + ext, x.budget - cc[e], x.pre_nodes + nd[e]};
-          if (cj[e] <= exact_item || y.base + ext0 > max_base) next_whole.push_back(y);
-          else next_split.push_back(y);
+          if (v[e] == -1) {   // the budget runs out inside item j (already mapped to path digits)
+            Job y{x.cand, (x.path << (3 * ext)) | j[e], x.base + ext, x.budget - cc[e], x.pre_n + nd[e],
+                  x.pre_c + cc[e], x.owner, x.slot, false};
+            y.whole = cj[e] <= exact_item || y.base + ext0 > max_base;
+            next_jobs.push_back(y);
+            continue;
+          }
+          // a capped item comes before any decision: the job becomes a frame
+          Frame F;
+          F.cand = x.cand; F.path = x.path; F.base = x.base; F.ext = ext; F.root_pair = root_pair;
+          F.budget = x.budget; F.pre_n = x.pre_n; F.pre_c = x.pre_c; F.owner = x.owner; F.slot = x.slot;
+          F.items = std::move(items[e]);
+          F.slot_of.assign(F.items.size(), -1);
+          const int f = (int)frames.size();
+          long long lower = 0;   // charges before each item, capped items counted as far as they got
+          for (size_t i = 0; i < F.items.size(); i++) {
+            const ItemRes &it = F.items[i];
+            if (it.v == 1 || it.v == 0 || it.v == 4 || it.v == 3) break;   // a decision: later items never matter
+            if (limited && lower + it.c > F.budget) break;                 // the budget runs out by here
+            if (it.v == 5) {
+              const int lo = 3 * (ext - 1);
+              const long long dig = root_pair ? (((6 | ((long long)i >> lo)) << lo) | ((long long)i & ((1ll << lo) - 1)))
+                                              : (long long)i;
+              Job y{x.cand, (x.path << (3 * ext)) | dig, x.base + ext, x.budget - lower, 0, 0, f, (int)F.subs.size(), false};
+              y.whole = y.base + ext0 > max_base;
+              F.slot_of[i] = (int)F.subs.size();
+              F.subs.push_back(Result{0, 0, 0, 0, 0, 0, false});
+              next_jobs.push_back(y);
+            }
+            lower += it.c;
+          }
+          frames.push_back(std::move(F));
+          resume(f);
         }
       }
     }
-    split.swap(next_split);
-    deep.swap(next_deep);
-    whole.swap(next_whole);
+    jobs.swap(next_jobs);
   }
   S->last_ms[1] = ms;
   long long units = 0;
   bool internal = false;
   for (int i = 0; i < ncand; i++) {
-    out[i].verdict = fv[i];
+    if (!fin[i].done) { set_error("completer: a candidate was left undecided (internal error)"); return WZFH_EKERNEL; }
+    out[i].verdict = fin[i].v;
     out[i].reserved = 0;
-    out[i].nodes = fnd[i];
-    out[i].A = fA[i];
-    out[i].B = fB[i];
-    units += fnd[i];
-    if (fv[i] == 4) internal = true;
+    out[i].nodes = fin[i].n;
+    out[i].A = fin[i].v == 0 || fin[i].v == 4 ? fin[i].A : 0;
+    out[i].B = fin[i].v == 0 || fin[i].v == 4 ? fin[i].B : 0;
+    units += fin[i].n;
+    if (fin[i].v == 4) internal = true;
   }
   S->last_units[1] = units;
+  {
+    std::lock_guard<std::mutex> lk(S->tot_mu);
+    S->tot_units[1] += units;
+  }
   if (internal) { set_error("completer FOUND failed the exact NPAF recheck"); return WZFH_EINTERNAL; }
+  return WZFH_OK;
+}
+
+// Per cell of a stream batch: stable-sort its candidates by flat score and take the first K
+// (the drain's stable_sort, wz_match.cpp:2654-2655, front-only top-K). Appends the chosen
+// candidates' packed words, owning cell slot, sorted rank and in-cell stream index.
+void select_topk(int n, const StreamOut &so, int ncells, int K, TopK &t) {
+  for (int i = 0; i < ncells; i++) {
+    const long long b = so.cell_begin[i], e = so.cell_begin[i + 1];
+    std::vector<std::pair<long long, long long>> key;
+    key.reserve(e - b);
+    for (long long j = b; j < e; j++) key.push_back({wzfh_flat_score(n, so.cands[j].X, so.cands[j].Y), j - b});
+    std::stable_sort(key.begin(), key.end(), [](auto &x, auto &y) { return x.first < y.first; });
+    const long long take = std::min<long long>(K, (long long)key.size());
+    for (long long r = 0; r < take; r++) {
+      const CandRec &c = so.cands[b + key[r].second];
+      t.cd.push_back(c.X); t.cd.push_back(c.Y);
+      t.owner.push_back(i); t.rank.push_back(r); t.sidx.push_back(key[r].second);
+    }
+  }
+}
+
+// The drain's counters per cell (wz_match.cpp:2686-2714): the sorted ranks are completed in
+// order and the drain stops at the first hit (or internal error, :2591-2595); ranks past it
+// are not counted. A cell's completions sit in rank order in the arrays.
+void aggregate_completions(const TopK &t, const wzfh_completion *comp, wzfh_search_result *res) {
+  for (size_t k = 0; k < t.owner.size(); k++) {
+    wzfh_search_result &r = res[t.owner[k]];
+    if (r.hit || r.internal) continue;
+    r.completed++;
+    r.ab_nodes += comp[k].nodes;
+    if (comp[k].verdict == 2) r.clean_no++;
+    if (comp[k].verdict == 3) r.aborted++;
+    if (comp[k].verdict == 4) r.internal = 1;
+    if (comp[k].verdict == 0) {
+      r.hit = 1; r.hit_rank = t.rank[k]; r.hit_idx = t.sidx[k];
+      r.A = comp[k].A; r.B = comp[k].B; r.C = t.cd[2 * k]; r.D = t.cd[2 * k + 1];
+    }
+  }
+}
+
+std::string stream_key(const wzfh_problem &P) {
+  return std::to_string(P.n) + "/" + std::to_string(P.m) + "/" + std::to_string(P.cfg.rev != 0) + "/" +
+         std::to_string(P.cfg.endpos != 0) + "/" + std::to_string(P.ab_side) + "/" + std::to_string(P.pinC) +
+         std::to_string(P.pinD);
+}
+
+int stream_handle(const wzfh_problem &P, wzfh_problem **out) {
+  *out = nullptr;
+  wzfh_problem *Q = new wzfh_problem();
+  Q->cfg = P.cfg;
+  Q->n = P.n; Q->L1 = P.L1; Q->m = P.m;
+  Q->pinC = P.pinC; Q->pinD = P.pinD; Q->pins_dropped = P.pins_dropped;
+  Q->ab_prof = false; Q->ab_side = P.ab_side;
+  std::memcpy(Q->hcos, P.hcos, sizeof Q->hcos);
+  std::memcpy(Q->hsin, P.hsin, sizeof Q->hsin);
+  int rc = device_init(*Q);
+  if (rc == WZFH_OK && P.dev) Q->dev->tune = P.dev->tune;
+  if (rc != WZFH_OK) { device_free(*Q); delete Q; return rc; }
+  *out = Q;
   return WZFH_OK;
 }
 
@@ -1033,61 +1337,46 @@ int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *
   return WZFH_OK;
 }
 
+int wzfh_kernel_totals(wzfh_problem *p, wzfh_kernel_sums *out, int reset) {
+  if (!p) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  DeviceState *S = p->dev;
+  std::lock_guard<std::mutex> lk(S->tot_mu);
+  for (int k = 0; k < 2; k++) {
+    if (out) { out->ms[k] = S->tot_ms[k]; out->units[k] = S->tot_units[k]; out->launches[k] = S->tot_launches[k]; }
+    if (reset) { S->tot_ms[k] = 0; S->tot_units[k] = 0; S->tot_launches[k] = 0; }
+  }
+  return WZFH_OK;
+}
+
 int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts,
                       wzfh_search_result *out) {
   if (!p || !cells || ncells <= 0 || !opts || !out) return WZFH_EINVAL;
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
   for (int i = 0; i < ncells; i++)
     if (cells[i] < 0 || cells[i] >= (int)p->cells.size()) { set_error("cell index out of range"); return WZFH_EINVAL; }
-  const int n = p->n;
   const int K = opts->top_k;
   auto one_batch = [&](const int *cl, int nc, wzfh_search_result *res) -> int {
     std::vector<wzfh_cell_stats> st(nc);
     StreamOut so;
     int rc = run_stream(*p, cl, nc, opts->stream, st.data(), so);
     if (rc != WZFH_OK) return rc;
-    // per cell: stable sort the streamed candidates by flat score, take the top K
-    std::vector<uint64_t> cd;
-    std::vector<int> cellof, owner;
-    std::vector<long long> rank, sidx;
+    TopK t;
+    select_topk(p->n, so, nc, K, t);
     for (int i = 0; i < nc; i++) {
-      const long long b = so.cell_begin[i], e = so.cell_begin[i + 1];
-      std::vector<std::pair<long long, long long>> key;
-      key.reserve(e - b);
-      for (long long j = b; j < e; j++) key.push_back({wzfh_flat_score(n, so.cands[j].X, so.cands[j].Y), j - b});
-      std::stable_sort(key.begin(), key.end(), [](auto &x, auto &y) { return x.first < y.first; });
-      const long long take = std::min<long long>(K, (long long)key.size());
-      for (long long r = 0; r < take; r++) {
-        const CandRec &c = so.cands[b + key[r].second];
-        cd.push_back(c.X); cd.push_back(c.Y);
-        cellof.push_back(cl[i]); owner.push_back(i); rank.push_back(r); sidx.push_back(key[r].second);
-      }
       res[i] = wzfh_search_result{};
       res[i].stream = st[i];
       res[i].hit_rank = -1; res[i].hit_idx = -1;
     }
-    const int nk = (int)owner.size();
+    const int nk = (int)t.owner.size();
+    std::vector<int> cellof(nk);
+    for (int k = 0; k < nk; k++) cellof[k] = cl[t.owner[k]];
     std::vector<wzfh_completion> comp(nk);
     const double stream_ms = p->dev->last_ms[0];
-    rc = run_complete(*p, cd.data(), cellof.data(), nk, opts->budget, comp.data());
+    rc = run_complete(*p, t.cd.data(), cellof.data(), nk, opts->budget, comp.data());
     p->dev->last_ms[0] = stream_ms;
     if (rc != WZFH_OK && rc != WZFH_EINTERNAL) return rc;
-    // the reference's drain completes the sorted ranks in order and stops at the first hit
-    // (wz_match.cpp:2686) or internal error (:2591-2595): ranks past it are not counted.
-    // A cell's completions sit in rank order in the launch's arrays.
-    for (int t = 0; t < nk; t++) {
-      wzfh_search_result &r = res[owner[t]];
-      if (r.hit || r.internal) continue;
-      r.completed++;
-      r.ab_nodes += comp[t].nodes;
-      if (comp[t].verdict == 2) r.clean_no++;
-      if (comp[t].verdict == 3) r.aborted++;
-      if (comp[t].verdict == 4) r.internal = 1;
-      if (comp[t].verdict == 0) {
-        r.hit = 1; r.hit_rank = rank[t]; r.hit_idx = sidx[t];
-        r.A = comp[t].A; r.B = comp[t].B; r.C = cd[2 * t]; r.D = cd[2 * t + 1];
-      }
-    }
+    aggregate_completions(t, comp.data(), res);
     return rc;
   };
   if (opts->batched) return one_batch(cells, ncells, out);
@@ -1128,7 +1417,8 @@ int wzfh_set_tuning(wzfh_problem *p, const wzfh_tuning *t) {
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
   if (t->inner_moves < 1 || t->inner_moves > 256 || t->donate_hops < 0 || t->donate_hops > 15 ||
       t->static_window < 1 || t->complete_split < 0 || t->complete_split > 6 || t->idle_sleep < 0 ||
-      t->idle_sleep > 256 || t->blocks_per_cu < 1 || t->blocks_per_cu > 4) {
+      t->idle_sleep > 256 || t->blocks_per_cu < 1 || t->blocks_per_cu > 4 || t->complete_cap_log2 < 0 ||
+      t->complete_cap_log2 > 40) {
     set_error("tuning value out of range");
     return WZFH_EINVAL;
   }

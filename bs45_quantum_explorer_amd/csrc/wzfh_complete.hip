@@ -29,6 +29,8 @@
 // ancestor level while unwinding a budget abort.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "wzfh_kernels.h"
 
 namespace wzfh {
@@ -539,8 +541,9 @@ namespace wzfh {
 // with an event decides: pre-reject (1), a hit (0/4: nodes up to and including the item),
 // the abort point (-1: the first item that stopped on its own budget or whose charges pass
 // the entry's budget; reported with the charges and nodes before it and its own charges), or
-// an item stopped at the launch's charge cap before any of those (-2: the entry is run again
-// split deeper). Every branch is on a ballot, so the loop exits uniformly.
+// an item stopped at the launch's charge cap before any of those (-2: the host scans the
+// entry's items and resolves the capped ones). A hit reports the nodes and charges through
+// it. Every branch is on a ballot, so the loop exits uniformly.
 __global__ __launch_bounds__(64) void complete_reduce_kernel(CompleteArgs a, int nent, int *fv, long long *fn,
                                                              uint64_t *fA, uint64_t *fB, int *fj, long long *fcc,
                                                              long long *fcur) {
@@ -576,8 +579,8 @@ __global__ __launch_bounds__(64) void complete_reduce_kernel(CompleteArgs a, int
       if (vf == 1) { v_out = 1; n_out = 0; c_out = cb; }
       else if (vf == 3 || (limited && cb + cf > bud)) { v_out = -1; j_out = (int)(p0 + f); n_out = nb; c_out = cb; cj_out = cf; }
       else if (vf == 5) { v_out = -2; n_out = nb; c_out = cb; }   // capped before a decision: split deeper
-      else {
-        v_out = vf; n_out = nb + nf; c_out = cb;
+      else {   // a hit: nodes and charges through the hit
+        v_out = vf; n_out = nb + nf; c_out = cb + cf;
         A = __shfl(in ? a.outA[u] : 0ull, f);
         B = __shfl(in ? a.outB[u] : 0ull, f);
       }
@@ -618,6 +621,18 @@ hipError_t launch_complete(const CompleteArgs &a, const uint8_t *kab, int blocks
   } else {
     hipLaunchKernelGGL(complete_kernel<false>, dim3(blocks), dim3(64), 0, st, a, kab);
   }
+  return hipGetLastError();
+}
+
+// Zero-fills n bytes (n a multiple of 8) with vector stores.
+__global__ void zero_kernel(uint64_t *p, size_t nwords) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t st) {
+  const size_t nw = (bytes + 7) / 8;
+  const int blocks = (int)std::min<size_t>(1024, std::max<size_t>(1, (nw + 255) / 256));
+  hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, st, (uint64_t *)p, nw);
   return hipGetLastError();
 }
 

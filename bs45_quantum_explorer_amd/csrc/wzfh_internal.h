@@ -74,8 +74,28 @@ struct StreamOut {
   std::vector<CandRec> cands;                 // by cell slot, then G
   std::vector<long long> cell_begin;          // per slot, into cands (ncells+1)
 };
+// cell_prob (optional): per cell slot, the problem whose cell list cells[i] indexes (problems
+// of one n with the same stream tables, see stream_key); null: every cell is P's.
 int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o, wzfh_cell_stats *stats,
-               StreamOut &out);
+               StreamOut &out, const wzfh_problem *const *cell_prob = nullptr);
+// The flattest K candidates of every cell of a stream batch, in sorted order per cell.
+struct TopK {
+  std::vector<uint64_t> cd;          // packed (C, D) pairs
+  std::vector<int> owner;            // cell slot
+  std::vector<long long> rank, sidx; // sorted rank, in-cell stream index
+};
+void select_topk(int n, const StreamOut &so, int ncells, int K, TopK &t);
+// Per-cell drain counters from the completions of t (results must hold the stream stats).
+void aggregate_completions(const TopK &t, const wzfh_completion *comp, wzfh_search_result *res);
+struct Carve;
+int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                 wzfh_completion *out, Carve *carve = nullptr);
+// Problems whose stream kernels use identical tables (same n, residue modulus, stream order,
+// ENDPOS, side and pinned roots) can share one stream launch.
+std::string stream_key(const wzfh_problem &P);
+// A device state for stream launches only, set up from P's tables (no cells, no rows): the
+// serving loop streams a group's cells on it while the problems' own states complete.
+int stream_handle(const wzfh_problem &P, wzfh_problem **out);
 // Batch size (cells) of one problem's search that fits its share of free device memory.
 long long serve_cell_slots(wzfh_problem &P, int nshare, const wzfh_search_opts &o);
 int problem_init(const wzfh_config &cfg, bool ab_side, bool with_device, wzfh_problem **out);

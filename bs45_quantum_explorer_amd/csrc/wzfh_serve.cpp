@@ -1,22 +1,28 @@
-// wzfh_serve.cpp -- cell-search serving loop with continuous batching.
+// wzfh_serve.cpp -- cell-search serving loop with continuous, in-flight batching.
 //
 // The reference has no server: its lanes are fixed arm processes started by the lane driver
-// (cluster/deploy/cluster_firsthit_probe.sh:100-107). This is a serving form of the same
-// cell unit (wzfh_search_cells): requests name a
-// problem (an (n, class) handle, set up once and kept device-resident) and a cell.
-// One worker thread per problem repeatedly admits whatever requests of its problem have
-// arrived -- up to max_batch cells and to the batch that fits its share of free device
-// memory -- and runs them as one persistent stream launch plus the completer launches on the
-// problem's own HIP stream. Requests that
-// arrive while a batch is on the device join the next batch of their problem as soon as
-// the current one returns, and batches of different problems run concurrently on their
-// streams, so the device work of mixed-n traffic overlaps. Results are identical to
-// running each cell alone: a cell's search depends only on the cell (the batched stream
-// is exact per cell, see wzfh_stream.hip), which tests/test_gpu_serve.py checks.
+// (cluster/deploy/cluster_firsthit_probe.sh:100-107). This is a serving form of the same cell
+// unit (wzfh_search_cells): requests name a problem (an (n, class) handle, set up once and kept
+// device-resident) and a cell.
+//
+// Problems whose stream kernels use the same tables (every class of one n under the same
+// flags) form a group. Each group has a stream stage: a thread with a device state of its own
+// that admits every request of the group that has arrived -- up to max_batch cells and to the
+// cells that fit the group's share of the device's free memory, re-read at every admission --
+// runs them as ONE persistent stream launch (cells of different classes side by side), picks
+// each cell's flattest top_k candidates, hands them to the problems' completion stages and
+// immediately admits the next requests. Each problem has a completion stage: a thread that
+// takes every stream batch queued for it and completes their candidates in one completer run
+// under the cells' profile rows on the problem's own HIP stream, then returns the results.
+// So a request that arrives while its group is streaming waits for that one launch only; the
+// completions of earlier batches run meanwhile on the other streams, and groups of different n
+// run concurrently. A result equals the same cell searched alone: a cell's stream is exact per
+// cell whatever else shares its launch, and its completions depend only on its candidates.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -26,63 +32,176 @@
 
 using Clock = std::chrono::steady_clock;
 
+namespace {
+
 struct ServeReq {
   long long ticket;
   int problem, cell;
   Clock::time_point submitted;
 };
 
+// One stream launch's share for one problem: its requests, their results (stream part set),
+// and their top-K candidates (owner = index into reqs).
+struct CompBatch {
+  std::vector<ServeReq> reqs;
+  std::vector<wzfh_search_result> res;
+  wzfh::TopK topk;
+  Clock::time_point started;
+  int launch_cells = 0;
+};
+
+struct Group {
+  std::vector<int> members;
+  wzfh_problem *sh = nullptr;      // stream handle (device state of the stream stage)
+  std::deque<ServeReq> pending;
+  bool stream_done = false;        // the stream stage has exited (closing)
+};
+
+}  // namespace
+
 struct wzfh_server {
   std::vector<wzfh_problem *> problems;
   wzfh_serve_opts opts;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
-  std::vector<std::deque<ServeReq>> pending;   // per problem
-  std::vector<long long> mem_slots;             // per problem: cells per batch that fit in memory
+  std::condition_variable cv_work, cv_comp, cv_done;
+  std::vector<Group> groups;
+  std::vector<int> group_of;                 // per problem
+  std::vector<std::deque<CompBatch>> comp_q; // per problem
   std::deque<wzfh_serve_result> done;
-  std::vector<std::thread> workers;
+  std::vector<std::thread> stream_threads, comp_threads;
   bool closing = false;
   long long next_ticket = 0;
   int error = WZFH_OK;
-  std::string error_msg;       // message of the first failing batch (error strings are per thread)
+  std::string error_msg;       // message of the first failure (error strings are per thread)
   Clock::time_point t0 = Clock::now();
+
+  double since(Clock::time_point t) const { return std::chrono::duration<double>(t - t0).count(); }
+
+  // Results of requests that failed before their completions (caller holds mu).
+  void fail(const std::vector<ServeReq> &reqs, int rc, const char *msg, Clock::time_point started, int launch) {
+    if (error == WZFH_OK) { error = rc; error_msg = msg; }
+    const Clock::time_point now = Clock::now();
+    for (const ServeReq &q : reqs) {
+      wzfh_serve_result r{};
+      r.ticket = q.ticket; r.problem = q.problem; r.cell = q.cell; r.status = rc; r.batch_size = launch;
+      r.submit_s = since(q.submitted); r.start_s = since(started); r.finish_s = since(now);
+      done.push_back(r);
+    }
+    cv_done.notify_all();
+  }
 };
 
-static void worker_loop(wzfh_server *S, int pidx) {
-  wzfh_problem *P = S->problems[pidx];
+static void stream_loop(wzfh_server *S, int g) {
+  Group &G = S->groups[g];
+  const int K = S->opts.search.top_k;
   while (true) {
+    const long long mem = wzfh::serve_cell_slots(*G.sh, (int)S->groups.size(), S->opts.search);
     std::vector<ServeReq> batch;
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv_work.wait(lk, [&] { return S->closing || !S->pending[pidx].empty(); });
-      if (S->pending[pidx].empty() && S->closing) return;
-      const long long slots = std::max(1ll, std::min((long long)S->opts.max_batch, S->mem_slots[pidx]));
-      while (!S->pending[pidx].empty() && (long long)batch.size() < slots) {
-        batch.push_back(S->pending[pidx].front());
-        S->pending[pidx].pop_front();
+      S->cv_work.wait(lk, [&] { return S->closing || !G.pending.empty(); });
+      if (G.pending.empty()) {
+        G.stream_done = true;
+        S->cv_comp.notify_all();
+        return;
+      }
+      const long long slots = std::max(1ll, std::min((long long)S->opts.max_batch, mem));
+      while (!G.pending.empty() && (long long)batch.size() < slots) {
+        batch.push_back(G.pending.front());
+        G.pending.pop_front();
       }
     }
-    std::vector<int> cells(batch.size());
-    for (size_t i = 0; i < batch.size(); i++) cells[i] = batch[i].cell;
-    std::vector<wzfh_search_result> res(batch.size());
+    const int nc = (int)batch.size();
+    std::vector<int> cells(nc);
+    std::vector<const wzfh_problem *> owner(nc);
+    for (int i = 0; i < nc; i++) { cells[i] = batch[i].cell; owner[i] = S->problems[batch[i].problem]; }
     const auto started = Clock::now();
-    int rc = wzfh_search_cells(P, cells.data(), (int)cells.size(), &S->opts.search, res.data());
-    const auto finished = Clock::now();
-    const char *msg = rc != WZFH_OK ? wzfh_last_error() : "";
+    std::vector<wzfh_cell_stats> st(nc);
+    wzfh::StreamOut so;
+    const int rc = wzfh::run_stream(*G.sh, cells.data(), nc, S->opts.search.stream, st.data(), so, owner.data());
+    if (rc != WZFH_OK) {
+      const std::string msg = wzfh_last_error();
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->fail(batch, rc, msg.c_str(), started, nc);
+      continue;
+    }
+    wzfh::TopK t;
+    wzfh::select_topk(G.sh->n, so, nc, K, t);
+    // split the launch by problem, keeping each cell's candidates in rank order
+    std::map<int, CompBatch> parts;
+    std::vector<int> local(nc);
+    for (int i = 0; i < nc; i++) {
+      CompBatch &b = parts[batch[i].problem];
+      local[i] = (int)b.reqs.size();
+      b.reqs.push_back(batch[i]);
+      wzfh_search_result r{};
+      r.stream = st[i];
+      r.hit_rank = -1; r.hit_idx = -1;
+      b.res.push_back(r);
+      b.started = started;
+      b.launch_cells = nc;
+    }
+    for (size_t k = 0; k < t.owner.size(); k++) {
+      CompBatch &b = parts[batch[t.owner[k]].problem];
+      b.topk.cd.push_back(t.cd[2 * k]); b.topk.cd.push_back(t.cd[2 * k + 1]);
+      b.topk.owner.push_back(local[t.owner[k]]);
+      b.topk.rank.push_back(t.rank[k]); b.topk.sidx.push_back(t.sidx[k]);
+    }
     std::lock_guard<std::mutex> lk(S->mu);
-    if (rc != WZFH_OK && rc != WZFH_EINTERNAL && S->error == WZFH_OK) { S->error = rc; S->error_msg = msg; }
-    for (size_t i = 0; i < batch.size(); i++) {
-      wzfh_serve_result r{};
-      r.ticket = batch[i].ticket;
-      r.problem = pidx;
-      r.cell = batch[i].cell;
-      r.status = rc;
-      r.batch_size = (int)batch.size();
-      r.result = res[i];
-      r.submit_s = std::chrono::duration<double>(batch[i].submitted - S->t0).count();
-      r.start_s = std::chrono::duration<double>(started - S->t0).count();
-      r.finish_s = std::chrono::duration<double>(finished - S->t0).count();
-      S->done.push_back(r);
+    for (auto &kv : parts) S->comp_q[kv.first].push_back(std::move(kv.second));
+    S->cv_comp.notify_all();
+  }
+}
+
+static void complete_loop(wzfh_server *S, int pidx) {
+  wzfh_problem *P = S->problems[pidx];
+  const int g = S->group_of[pidx];
+  while (true) {
+    std::vector<CompBatch> batches;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_comp.wait(lk, [&] { return !S->comp_q[pidx].empty() || S->groups[g].stream_done; });
+      if (S->comp_q[pidx].empty()) return;
+      while (!S->comp_q[pidx].empty()) {
+        batches.push_back(std::move(S->comp_q[pidx].front()));
+        S->comp_q[pidx].pop_front();
+      }
+    }
+    // every queued batch's candidates in one completer run
+    std::vector<uint64_t> cd;
+    std::vector<int> cell_of;
+    for (const CompBatch &b : batches) {
+      cd.insert(cd.end(), b.topk.cd.begin(), b.topk.cd.end());
+      for (int o : b.topk.owner) cell_of.push_back(b.reqs[o].cell);
+    }
+    const int nk = (int)cell_of.size();
+    std::vector<wzfh_completion> comp(nk);
+    int rc = nk ? wzfh::run_complete(*P, cd.data(), cell_of.data(), nk, S->opts.search.budget, comp.data())
+                : WZFH_OK;
+    const std::string msg = rc != WZFH_OK ? wzfh_last_error() : "";
+    const auto finished = Clock::now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (rc != WZFH_OK && rc != WZFH_EINTERNAL) {
+      for (const CompBatch &b : batches) S->fail(b.reqs, rc, msg.c_str(), b.started, b.launch_cells);
+      continue;
+    }
+    size_t at = 0;
+    for (CompBatch &b : batches) {
+      wzfh::aggregate_completions(b.topk, comp.data() + at, b.res.data());
+      at += b.topk.owner.size();
+      for (size_t i = 0; i < b.reqs.size(); i++) {
+        wzfh_serve_result r{};
+        r.ticket = b.reqs[i].ticket;
+        r.problem = pidx;
+        r.cell = b.reqs[i].cell;
+        r.status = (rc == WZFH_EINTERNAL && b.res[i].internal) ? WZFH_EINTERNAL : WZFH_OK;
+        r.batch_size = b.launch_cells;
+        r.result = b.res[i];
+        r.submit_s = S->since(b.reqs[i].submitted);
+        r.start_s = S->since(b.started);
+        r.finish_s = S->since(finished);
+        S->done.push_back(r);
+      }
     }
     S->cv_done.notify_all();
   }
@@ -97,9 +216,29 @@ int wzfh_serve_open(wzfh_problem **problems, int nproblems, const wzfh_serve_opt
   wzfh_server *S = new wzfh_server();
   S->problems.assign(problems, problems + nproblems);
   S->opts = *opts;
-  S->pending.resize(nproblems);
-  for (int i = 0; i < nproblems; i++) S->mem_slots.push_back(wzfh::serve_cell_slots(*problems[i], nproblems, opts->search));
-  for (int i = 0; i < nproblems; i++) S->workers.emplace_back(worker_loop, S, i);
+  S->comp_q.resize(nproblems);
+  S->group_of.assign(nproblems, -1);
+  std::map<std::string, int> by_key;
+  for (int i = 0; i < nproblems; i++) {
+    const std::string k = wzfh::stream_key(*problems[i]);
+    auto it = by_key.find(k);
+    if (it == by_key.end()) {
+      it = by_key.emplace(k, (int)S->groups.size()).first;
+      S->groups.emplace_back();
+    }
+    S->group_of[i] = it->second;
+    S->groups[it->second].members.push_back(i);
+  }
+  for (Group &G : S->groups) {
+    const int rc = wzfh::stream_handle(*problems[G.members[0]], &G.sh);
+    if (rc != WZFH_OK) {
+      for (Group &H : S->groups) if (H.sh) wzfh_destroy(H.sh);
+      delete S;
+      return rc;
+    }
+  }
+  for (int g = 0; g < (int)S->groups.size(); g++) S->stream_threads.emplace_back(stream_loop, S, g);
+  for (int i = 0; i < nproblems; i++) S->comp_threads.emplace_back(complete_loop, S, i);
   *out = S;
   return WZFH_OK;
 }
@@ -111,7 +250,7 @@ int wzfh_serve_submit(wzfh_server *S, int problem, int cell, long long *ticket) 
   if (S->closing) return WZFH_EINVAL;
   ServeReq r{S->next_ticket++, problem, cell, Clock::now()};
   if (ticket) *ticket = r.ticket;
-  S->pending[problem].push_back(r);
+  S->groups[S->group_of[problem]].pending.push_back(r);
   S->cv_work.notify_all();
   return WZFH_OK;
 }
@@ -134,8 +273,11 @@ int wzfh_serve_close(wzfh_server *S) {
     std::lock_guard<std::mutex> lk(S->mu);
     S->closing = true;
     S->cv_work.notify_all();
+    S->cv_comp.notify_all();
   }
-  for (auto &t : S->workers) t.join();
+  for (auto &t : S->stream_threads) t.join();
+  for (auto &t : S->comp_threads) t.join();
+  for (Group &G : S->groups) if (G.sh) wzfh_destroy(G.sh);
   const int rc = S->error;
   delete S;
   return rc;

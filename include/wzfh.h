@@ -196,7 +196,8 @@ typedef struct {
   int idle_sleep;         /* stream: 64-cycle sleeps of a wave with nothing to walk (0..256)*/
   int blocks_per_cu;      /* stream: persistent blocks (4 waves each) per compute unit (1..4)*/
   int trials;             /* wzfh_autotune: configurations timed                           */
-  int reserved;
+  int complete_cap_log2;  /* completer: split items stop after 2^this charges and their
+                             subtrees are split again (0 = no cap)                        */
   double stream_ms;       /* wzfh_autotune: stream kernel time of the chosen settings      */
   double complete_ms;     /* wzfh_autotune: completer time of the chosen settings          */
 } wzfh_tuning;
@@ -285,6 +286,18 @@ long long wzfh_flat_score(int n, uint64_t C, uint64_t D);
  * 8 donations, 9 donated work items, 11 lane DFS moves, 12 wave passes of the main loop.
  * kind 10: the leaf test's margin. */
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units);
+/* Running totals of a handle since the last reset (or since wzfh_init), for measurement over
+ * a timed region: summed HIP-event durations of the stream kernel's launches (index 0) and of
+ * the completer kernel's launches without their per-entry reduce (index 1), the units they
+ * processed (counted C,D DFS nodes; completer nodes) and the launch counts. reset != 0 zeroes
+ * the totals after reading them. No reference counterpart (the reference times phases with
+ * host clocks, wz_match.cpp:2532-2553). */
+typedef struct {
+  double ms[2];
+  long long units[2];
+  long long launches[2];
+} wzfh_kernel_sums;
+int wzfh_kernel_totals(wzfh_problem *p, wzfh_kernel_sums *out, int reset);
 
 #ifdef __cplusplus
 }

@@ -424,19 +424,34 @@ struct Stream {
   }
   // hall_ok_pair_fast decision (wz_match.cpp:143-168): reject iff some angle j has
   // P_X > lim+0.5, P_Y > lim+0.5 or P_X+P_Y > lim+0.5, with P = rx*rx + ix*ix and the
-  // sums accumulated left to right in FP64. Angle order does not change the decision.
+  // sums accumulated left to right in FP64. The decision is a conjunction over angles, so
+  // the visiting order does not change it; like the reference, angles are visited most-
+  // rejecting first (per thread, re-sorted every 4096 rejections, :158-162).
+  struct HallOrder {
+    int order[200];
+    long long fail[201] = {0}, rej = 0;
+    HallOrder() { for (int t = 0; t < 200; t++) order[t] = t + 1; }
+  };
+  static HallOrder &hall_order() { static thread_local HallOrder h; return h; }
   bool hall() const {
     const double lim = 4.0 * P.n + 2.0 + 0.5;
-    for (int j = 1; j <= 200; j++) {
+    HallOrder &H = hall_order();
+    for (int t = 0; t < 200; t++) {
+      const int j = H.order[t];
       const double *cj = P.hcos[j], *sj = P.hsin[j];
       double rx = 0, ix = 0, ry = 0, iy = 0;
       for (int i = 0; i < L; i++) { rx += X[i] * cj[i]; ix += X[i] * sj[i]; }
-      double px2 = rx * rx + ix * ix;
-      if (px2 > lim) return false;
-      for (int i = 0; i < L; i++) { ry += Y[i] * cj[i]; iy += Y[i] * sj[i]; }
-      double py2 = ry * ry + iy * iy;
-      if (py2 > lim) return false;
-      if (rx * rx + ix * ix + ry * ry + iy * iy > lim) return false;
+      bool fail = rx * rx + ix * ix > lim;
+      if (!fail) {
+        for (int i = 0; i < L; i++) { ry += Y[i] * cj[i]; iy += Y[i] * sj[i]; }
+        fail = ry * ry + iy * iy > lim || rx * rx + ix * ix + ry * ry + iy * iy > lim;
+      }
+      if (fail) {
+        H.fail[j]++;
+        if ((++H.rej & 4095) == 0)
+          std::stable_sort(H.order, H.order + 200, [&](int a, int b) { return H.fail[a] > H.fail[b]; });
+        return false;
+      }
     }
     return true;
   }

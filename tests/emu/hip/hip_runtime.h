@@ -134,6 +134,11 @@ inline hipError_t hipMalloc(void **p, size_t b) {
   return *p ? 0 : 1;
 }
 inline hipError_t hipFree(void *p) { free(p); return 0; }
+#ifndef __align__
+#define __align__(n) __attribute__((aligned(n)))
+#endif
+#define HIP_SYMBOL(X) X
+template <typename T> hipError_t hipGetSymbolAddress(void **p, const T &sym) { *p = (void *)&sym; return 0; }
 inline hipError_t hipMemGetInfo(size_t *free_b, size_t *total_b) { *free_b = size_t(8) << 30; *total_b = size_t(16) << 30; return 0; }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t b, int) { memcpy(d, s, b); return 0; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t b, int, hipStream_t) { memcpy(d, s, b); return 0; }

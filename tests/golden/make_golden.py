@@ -322,6 +322,27 @@ def gen_join(jobs):
           f'{sum(r["cd_stream"] for r in recs)} C,D candidates, {sum(r["hits"] for r in recs)} hits')
 
 
+def signed_variants(sig):
+    """Every signing of a class representative (a, b, c, d) that keeps its norm: the join
+    searches one SIGNED signature, and a negative is "no 2.2-normalized solution with this
+    signed sig" (wz_match.cpp:2900-2904)."""
+    out = []
+    for mask in range(16):
+        v = tuple(-x if (mask >> i) & 1 else x for i, x in enumerate(sig))
+        if v not in out:
+            out.append(v)
+    return out
+
+
+def gen_join_signed(jobs):
+    tasks = [(n, v, {}) for n in range(6, 13) for sig in classes(n) for v in signed_variants(sig)[1:]]
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: join_case(*t), tasks))
+    (HERE / 'joins_signed.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'joins_signed.json: {len(recs)} runs, {sum(r["found"] for r in recs)} found, '
+          f'{sum(r["exhausted"] for r in recs)} exhausted')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--only', default='')
@@ -330,7 +351,7 @@ def main():
     if not REF.exists():
         sys.exit(f'{REF} missing: run make -C oracle ref (needs the reference tree)')
     gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29,
-                solutions=gen_solutions, join=gen_join)
+                solutions=gen_solutions, join=gen_join, join_signed=gen_join_signed)
     for name, g in gens.items():
         if not a.only or name in a.only.split(','):
             g(a.jobs)
