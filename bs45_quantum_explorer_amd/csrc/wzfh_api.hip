@@ -32,6 +32,30 @@ hipError_t launch_complete_reduce(const CompleteArgs &a, int nent, int *fv, long
 hipError_t complete_set_lds_limit(size_t bytes);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t st);
 
+// ---------------------------------------------------------------------------
+// Allocation-free completer (wzfh_complete_noalloc): every per-call device buffer of that entry
+// point is carved from this file-scope array. The loader gives each GPU its own copy, held
+// until the process ends; its address on a device comes from hipGetSymbolAddress (in host code
+// the name `scratch` is only a host-side stand-in, never a device address).
+static __device__ __align__(256) unsigned char scratch[WZFH_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[256];   // per device ordinal: the array's device address
+
+// Looks up (once per device, with that device current) the device address of `scratch`.
+static int scratch_address(int device, void **out) {
+  if (device < 0 || device >= 256) return WZFH_EINVAL;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return WZFH_EDEVICE;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return WZFH_OK;
+}
+
 static thread_local std::string g_err;
 std::string set_error(const std::string &msg) { g_err = msg; return msg; }
 
@@ -89,6 +113,7 @@ struct DeviceState {
   std::mutex tot_mu;
   double tot_ms[2] = {0, 0};
   long long tot_units[2] = {0, 0}, tot_launches[2] = {0, 0};
+  void *noalloc = nullptr;         // this device's copy of `scratch` (allocation-free completer)
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
   unsigned long long last_telem[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   double hall_eps = 0.0;           // margin of the matrix-core leaf test
@@ -272,6 +297,10 @@ int device_init(wzfh_problem &P) {
   if (S->complete_lds_max > 0) WZ_HIP(complete_set_lds_limit(S->complete_lds_max));
   WZ_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
   for (auto &e : S->ev) WZ_HIP(hipEventCreate(&e));
+  if (scratch_address(S->device, &S->noalloc) != WZFH_OK) {
+    set_error("hipGetSymbolAddress of the allocation-free scratch array failed");
+    return WZFH_EDEVICE;
+  }
   const size_t tb = 201 * 64 * sizeof(double);
   if (S->hcos.ensure(tb) || S->hsin.ensure(tb)) return WZFH_ENOMEM;
   WZ_HIP(hipMemcpy(S->hcos.p, P.hcos, tb, hipMemcpyHostToDevice));
@@ -1293,6 +1322,21 @@ int wzfh_complete(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int n
   if (!p || (ncand > 0 && (!cd || !out)) || ncand < 0 || budget < 0) return WZFH_EINVAL;
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
   return run_complete(*p, cd, cell_of, ncand, budget, out);
+}
+
+int wzfh_complete_noalloc(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                          wzfh_completion *out) {
+  if (!p || !cd || ncand < 0 || !out) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  if (ncand > WZFH_NOALLOC_MAX_CAND) {
+    set_error("wzfh_complete_noalloc: at most " + std::to_string(WZFH_NOALLOC_MAX_CAND) + " candidates per call");
+    return WZFH_EINVAL;
+  }
+  if (!p->dev->noalloc) { set_error("allocation-free scratch not set up on this device"); return WZFH_EDEVICE; }
+  Carve carve;
+  carve.base = (char *)p->dev->noalloc;
+  carve.cap = WZFH_NOALLOC_SCRATCH_BYTES;
+  return run_complete(*p, cd, cell_of, ncand, budget, out, &carve);
 }
 
 long long wzfh_flat_score(int n, uint64_t C, uint64_t D) {

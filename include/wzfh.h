@@ -141,6 +141,20 @@ typedef struct {
 int wzfh_complete(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand,
                   long long budget, wzfh_completion *out);
 
+/* Allocation-free variant of wzfh_complete. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, kernels and results as wzfh_complete, but
+ * every per-call device buffer (work items, per-entry reduce, counters, row scratch) lives in
+ * one static device array of WZFH_NOALLOC_SCRATCH_BYTES that each GPU holds until the process
+ * ends; the call allocates and frees no device memory and clears what must start at zero with
+ * a fill kernel. At most WZFH_NOALLOC_MAX_CAND candidates per call (else WZFH_EINVAL); a
+ * launch that would not fit the array returns WZFH_ECAPACITY. The array keeps the last call's
+ * contents and is shared by every handle on that GPU: calls that overlap on one GPU race. */
+#define WZFH_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define WZFH_NOALLOC_MAX_CAND 1024
+int wzfh_complete_noalloc(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                          wzfh_completion *out);
+
 /* ---- Front-only cell search: stream -> flatness top-K -> complete ------------
  * The unit of the reference's front policy (WZ_FH_BUF_CAP=cand_cap,
  * WZ_FH_DRAIN_TOP=top_k, WZ_FH_DRAIN_BATCHES=1; wz_match.cpp:2656-2714), batched over

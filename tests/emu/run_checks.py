@@ -42,6 +42,18 @@ for split, cap, c in plan:
             ok = list(wz.unpack(o.A, 14)) == c['A'] and list(wz.unpack(o.B, 14)) == c['B']
         report(ok, 'complete', c['name'], 'budget', c['budget'], 'split', split, 'cap', cap, (o.verdict, o.nodes),
                (c['r'], c['nodes']))
+# the allocation-free entry point: the same completions with every buffer carved from the
+# library's static scratch array (two calls in a row, the second smaller)
+os.environ['WZFH_COMPLETE_SPLIT'] = '2'
+os.environ['WZFH_COMPLETE_ITEM_CAP'] = '0'
+for batch in (cases, cases[:2]):
+    cd = np.array([[wz.pack(c['C']), wz.pack(c['D'])] for c in batch], dtype=np.uint64)
+    for budget in sorted({c['budget'] for c in batch}):
+        mine = [c for c in batch if c['budget'] == budget]
+        sub = np.array([[wz.pack(c['C']), wz.pack(c['D'])] for c in mine], dtype=np.uint64)
+        got = P.complete(sub, budget=budget, noalloc=True)
+        ok = [(o.verdict, o.nodes) for o in got] == [(c['r'], c['nodes']) for c in mine]
+        report(ok, 'complete_noalloc', len(batch), 'budget', budget, [(o.verdict, o.nodes) for o in got])
 P.close()
 
 # stream: per-cell counts and stream hash of small-n runs, with donation forced every 2 moves
