@@ -13,7 +13,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <unordered_map>
+#include <string>
 #include <unordered_set>
+#include <utility>
+#include <vector>
 
 #include "wzfh_internal.h"
 
@@ -225,15 +228,34 @@ struct AutoKeys {
   bool has(const int *T, int nT, int need) const { return keys.count(pack_auto(T, nT, need)) != 0; }
 };
 
-// Integer cell key ordered exactly like the reference's string key
-// "v,v,...,|v,v,...," (wz_match.cpp:1665-1671) compared with std::string operator<:
-// for single-digit entries the first differing entry decides, negatives come first
-// ordered by |v| ascending ('-' < '0'), then 0,1,2,... (|v| <= 9 holds for n <= 53).
-uint64_t elem_rank(int v) { return v < 0 ? (uint64_t)(-v - 1) : (uint64_t)(9 + v); }
-uint64_t cell_rank_key(const int8_t *px, const int8_t *py, int m) {
-  uint64_t k = 0;
-  for (int c = 0; c < m; c++) k = (k << 5) | elem_rank(px[c]);
-  for (int c = 0; c < m; c++) k = (k << 5) | elem_rank(py[c]);
+// Integer cell key ordered exactly like the reference's string key "v,v,...,|v,v,...,"
+// (wz_match.cpp:1665-1671) under std::string operator<. Every entry is the text "v," and
+// those texts are prefix-free (the comma ends each), so comparing two keys is comparing their
+// entry sequences lexicographically, each entry by the string order of its text. An entry's
+// rank is its text's position among the texts of -31..31 (cells have |v| <= 11 for n <= 63;
+// the quad switch Q maps a cell to entries of up to twice that, e.g. -16..16 at n = 44, where
+// "-1," < "-10," < "-16," < "-2," < ... < "0," < "1," < "10," < ... < "9,"). Six bits per entry,
+// 12 entries at m = 6: a 128-bit key.
+struct ElemRank {
+  uint8_t r[63];
+  ElemRank() {
+    std::vector<std::pair<std::string, int>> t;
+    for (int v = -31; v <= 31; v++) t.push_back({std::to_string(v) + ",", v});
+    std::sort(t.begin(), t.end());
+    for (size_t i = 0; i < t.size(); i++) r[t[i].second + 31] = (uint8_t)i;
+  }
+};
+const ElemRank kElemRank;
+using CellKey = unsigned __int128;
+struct CellKeyHash {
+  size_t operator()(CellKey k) const {
+    return std::hash<uint64_t>()((uint64_t)k ^ ((uint64_t)(k >> 64) * 0x9E3779B97F4A7C15ull));
+  }
+};
+CellKey cell_rank_key(const int8_t *px, const int8_t *py, int m) {
+  CellKey k = 0;
+  for (int c = 0; c < m; c++) k = (k << 6) | kElemRank.r[px[c] + 31];
+  for (int c = 0; c < m; c++) k = (k << 6) | kElemRank.r[py[c] + 31];
   return k;
 }
 std::string cell_string(const int8_t *px, const int8_t *py, int m) {
@@ -407,12 +429,12 @@ int build_problem(wzfh_problem &P) {
     bool orbit_q = (c.orbit_q == 2) || (c.orbit_q == 1 && q_safe);
     int nvar = orbit_q ? 64 : 32;
     auto revp = [&](const int8_t *p, int8_t *q) { for (int k = 0; k < m; k++) q[((n - 1 - k) % m + m) % m] = p[k]; };
-    std::vector<uint64_t> own(cells.size()), oid(cells.size());
-    std::unordered_map<uint64_t, uint64_t> orbit_min;
+    std::vector<CellKey> own(cells.size()), oid(cells.size());
+    std::unordered_map<CellKey, CellKey, CellKeyHash> orbit_min;
     orbit_min.reserve(cells.size());
     for (size_t i = 0; i < cells.size(); i++) {
       own[i] = cell_rank_key(cells[i].px, cells[i].py, m);
-      uint64_t best = ~0ull;
+      CellKey best = ~(CellKey)0;
       for (int var = 0; var < nvar; var++) {
         int8_t x[8], y[8], t[8];
         std::memcpy(x, cells[i].px, 8); std::memcpy(y, cells[i].py, 8);
@@ -426,7 +448,7 @@ int build_problem(wzfh_problem &P) {
         if (var & 2) for (int k = 0; k < m; k++) y[k] = (int8_t)-y[k];
         if (var & 4) { revp(x, t); std::memcpy(x, t, 8); }
         if (var & 8) { revp(y, t); std::memcpy(y, t, 8); }
-        uint64_t k = (var & 16) ? cell_rank_key(y, x, m) : cell_rank_key(x, y, m);
+        const CellKey k = (var & 16) ? cell_rank_key(y, x, m) : cell_rank_key(x, y, m);
         best = std::min(best, k);
       }
       oid[i] = best;
@@ -434,7 +456,7 @@ int build_problem(wzfh_problem &P) {
       if (it == orbit_min.end() || own[i] < it->second) orbit_min[best] = own[i];
     }
     if (c.order == 3) {   // Pass H total order (wz_match.cpp:1816-1856)
-      std::unordered_map<uint64_t, long long> omin;
+      std::unordered_map<CellKey, long long, CellKeyHash> omin;
       for (size_t i = 0; i < cells.size(); i++) {
         long long sc = prof_score(cells[i], m);
         auto it = omin.find(oid[i]);
@@ -449,10 +471,10 @@ int build_problem(wzfh_problem &P) {
         return own[x] < own[y];
       });
       std::vector<HostCell> sc(cells.size());
-      std::vector<uint64_t> own2(cells.size()), oid2(cells.size());
+      std::vector<CellKey> own2(cells.size()), oid2(cells.size());
       for (size_t i = 0; i < perm.size(); i++) { sc[i] = cells[perm[i]]; own2[i] = own[perm[i]]; oid2[i] = oid[perm[i]]; }
       cells.swap(sc); own.swap(own2); oid.swap(oid2);
-      std::unordered_set<uint64_t> uniq(own.begin(), own.end());
+      std::unordered_set<CellKey, CellKeyHash> uniq(own.begin(), own.end());
       if (uniq.size() != own.size()) { set_error("duplicate cell keys in the raw list"); return WZFH_EINVAL; }
       uint64_t dl = 1469598103934665603ULL, dk = 1469598103934665603ULL;
       auto fnv = [](uint64_t h, const std::string &k) { for (unsigned char ch : k) { h ^= ch; h *= 1099511628211ULL; } return h; };

@@ -40,3 +40,20 @@ def test_join_cd_profiles_match_reference():
         P.close()
         if r['found']:
             assert wz.npaf_max(r['n'], r['A'], r['B'], r['C'], r['D']) == 0
+
+
+ORDER3 = load('order3_digests.json')
+
+
+@pytest.mark.parametrize('n', sorted({r['n'] for r in ORDER3}))
+def test_order3_list_digest_matches_reference(n):
+    """WZ_FH_PROF_ORDER=3 (wz_match.cpp:1809-1872): the total order of the cell list under the
+    orbit canon (32- or 64-element group) gives the reference's cell count, kept-orbit count and
+    both FNV-1a digests ("<list>:<kept>") that WZ_FH_EXPECT_DIGEST checks."""
+    for r in (x for x in ORDER3 if x['n'] == n):
+        st = r['settings']
+        P = wz.Problem(n, r['sig'], device=False, m6=1, canon=1, order=3, orbit_q=r['orbit_q'],
+                       thm211b=int(st.get('WZ_THM211B', 0)), thm212=int(st.get('WZ_THM212', 0)))
+        kept = sum(P.cell(pi)[2] for pi in range(P.num_cells()))
+        assert (P.num_cells(), kept, P.digest()) == (r['cells'], r['kept'], r['digest']), (n, r['sig'], r['orbit_q'])
+        P.close()
