@@ -22,7 +22,8 @@ EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy',
            'wzfh_search_cells', 'wzfh_npaf_check', 'wzfh_flat_score', 'wzfh_last_kernel_ms',
            'wzfh_serve_open', 'wzfh_serve_submit', 'wzfh_serve_poll', 'wzfh_serve_close',
            'wzfh_join_open', 'wzfh_join_build', 'wzfh_join_probe', 'wzfh_join_close',
-           'wzfh_autotune', 'wzfh_get_tuning', 'wzfh_set_tuning', 'wzfh_kernel_totals', 'wzfh_complete_noalloc')
+           'wzfh_autotune', 'wzfh_get_tuning', 'wzfh_set_tuning', 'wzfh_kernel_totals', 'wzfh_complete_noalloc',
+           'wzfh_serve_kernel_totals', 'wzfh_join_solutions', 'wzfh_qprune_stats')
 NOALLOC_MAX_CAND = 1024            # WZFH_NOALLOC_MAX_CAND
 NOALLOC_SCRATCH_BYTES = 64 << 20   # WZFH_NOALLOC_SCRATCH_BYTES
 
@@ -37,7 +38,7 @@ class WzfhError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [(k, C.c_int) for k in ('n', 'a', 'b', 'c', 'd', 'm6', 'thm211b', 'thm212', 'canon', 'orbit_q',
                                         'endpos', 'order', 'rev', 'ab_prof', 'outer_k', 'no_canon_ab')] + \
-               [('abp_cap', C.c_longlong)]
+               [('abp_cap', C.c_longlong), ('orbit_qprune', C.c_int), ('reserved', C.c_int)]
 
 
 class StreamOpts(C.Structure):
@@ -123,6 +124,7 @@ def lib():
         L.wzfh_join_open.argtypes = [C.POINTER(Config), C.POINTER(vp)]
         L.wzfh_join_build.argtypes = [vp, C.c_int, vp]
         L.wzfh_join_probe.argtypes = [vp, vp]
+        L.wzfh_join_solutions.argtypes = [vp, vp, C.c_longlong, C.POINTER(C.c_longlong)]
         L.wzfh_join_close.argtypes = [vp]
         L.wzfh_join_close.restype = None
         L.wzfh_last_kernel_ms.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
@@ -134,6 +136,7 @@ def lib():
         L.wzfh_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.wzfh_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.wzfh_kernel_totals.argtypes = [vp, C.POINTER(KernelTotals), C.c_int]
+        L.wzfh_serve_kernel_totals.argtypes = [vp, C.POINTER(KernelTotals), C.c_int]
         _lib = L
     return _lib
 
@@ -202,6 +205,13 @@ class Problem:
         out = np.zeros(max(cnt, 1), dtype=np.int32)
         lib().wzfh_live_cells(self.h, out.ctypes.data, cnt)
         return [int(x) for x in out[:cnt]]
+
+    def qprune_stats(self):
+        """(ran, [missing Q images, not realizable, eq 2.12, nonintegral, unknown, dead orbits,
+        cells removed]) of the Q-closure prune."""
+        st = (C.c_longlong * 7)()
+        ran = lib().wzfh_qprune_stats(self.h, st)
+        return ran == 1, list(st)
 
     def digest(self):
         buf = C.create_string_buffer(64)
@@ -306,6 +316,12 @@ class Server:
             _check(k)
         return [out[i] for i in range(k)]
 
+    def kernel_totals(self, reset=False):
+        """(ms, units, launches) per kernel (0 stream, 1 completer) over the server's launches."""
+        t = KernelTotals()
+        _check(lib().wzfh_serve_kernel_totals(self.h, C.byref(t), int(reset)))
+        return [(t.ms[k], t.units[k], t.launches[k]) for k in range(2)]
+
     def close(self):
         if getattr(self, 'h', None):
             rc = lib().wzfh_serve_close(self.h)
@@ -340,6 +356,14 @@ class Join:
     def probe(self):
         _check(lib().wzfh_join_probe(self.h, C.byref(self.result)))
         return self.result
+
+    def solutions(self):
+        """Every solution found by the probe, as a sorted list of packed (A, B, C, D) word tuples."""
+        cnt = C.c_longlong()
+        _check(lib().wzfh_join_solutions(self.h, None, 0, C.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 4), np.uint64)
+        _check(lib().wzfh_join_solutions(self.h, out.ctypes.data, cnt.value, C.byref(cnt)))
+        return [tuple(int(v) for v in row) for row in out[:cnt.value]]
 
     def close(self):
         if getattr(self, 'h', None):

@@ -709,9 +709,12 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   for (const CompleteEntry &x : ents) root_pair = root_pair && x.base == 0;
   const int items_log2 = 3 * ext - (root_pair ? 2 : 0);
   const long long nwork = (long long)nent << items_log2;
-  int blocks = (int)std::max(1ll, std::min((long long)S->num_cu * 8, nwork));
+  int bpc = 8;
+  if (const char *e = getenv("WZFH_COMPLETE_BPC")) bpc = std::max(1, std::min(32, atoi(e)));
+  int blocks = (int)std::max(1ll, std::min((long long)S->num_cu * bpc, nwork));
   const size_t lds_bytes = complete_lds_bytes(P.L1, words);
-  const bool lds = lds_bytes <= S->complete_lds_max;
+  bool lds = lds_bytes <= S->complete_lds_max;
+  if (const char *e = getenv("WZFH_COMPLETE_LDS")) lds = lds && atoi(e) != 0;
   const size_t nw = (size_t)nwork;
   if (carve && !lds) {   // row bitsets in global memory: as many waves as the carve holds
     const size_t fixed = kCarveSlack + (size_t)nent * kEntryBytes + nw * kItemBytes;
@@ -1274,6 +1277,12 @@ int wzfh_live_cells(const wzfh_problem *p, int *out, int cap) {
     cnt++;
   }
   return cnt;
+}
+
+int wzfh_qprune_stats(const wzfh_problem *p, long long *stats) {
+  if (!p) return WZFH_EINVAL;
+  if (stats) for (int k = 0; k < 7; k++) stats[k] = p->qp_stats[k];
+  return p->qp_active ? 1 : 0;
 }
 
 int wzfh_order_digest(const wzfh_problem *p, char *buf, int buflen) {

@@ -59,6 +59,8 @@ struct wzfh_problem {
   std::vector<int> row_off, row_len;   // per row list
   std::vector<wzfh::AbRow> rows;
   std::string digest;                  // order-3 "<list>:<kept>"
+  bool qp_active = false;              // the Q-closure prune ran
+  long long qp_stats[7] = {0, 0, 0, 0, 0, 0, 0};
   double hcos[201][wzfh::kMaxL];
   double hsin[201][wzfh::kMaxL];
   wzfh::DeviceState *dev = nullptr;

@@ -163,9 +163,10 @@ int main(int argc, char **argv) {
   cfg.no_canon_ab = env_set("WZ_FH_NO_CANON");
   cfg.abp_cap = env_ll("WZ_FH_AB_PROF_CAP", 20000);
   if (const char *e = getenv("WZ_FH_AB_PROF")) cfg.ab_prof = atoi(e) != 0;
-  if (env_ll("WZ_FH_CD_PRUNE", 0) || env_ll("WZ_FH_A2", 0) || env_ll("WZ_FH_ORBIT_QPRUNE", 0) ||
-      env_ll("WZ_FH_WALL_SEC", 0) || env_set("WZ_FH_TELEMETRY") || env_set("WZ_FH_TARGET_C")) {
-    printf("RESULT: UNSUPPORTED FLAG (CD_PRUNE, A2, ORBIT_QPRUNE, WALL_SEC, TELEMETRY, TARGET are not implemented)\n");
+  cfg.orbit_qprune = env_ll("WZ_FH_ORBIT_QPRUNE", 0) != 0;
+  if (env_ll("WZ_FH_CD_PRUNE", 0) || env_ll("WZ_FH_A2", 0) || env_ll("WZ_FH_WALL_SEC", 0) ||
+      env_set("WZ_FH_TELEMETRY") || env_set("WZ_FH_TARGET_C")) {
+    printf("RESULT: UNSUPPORTED FLAG (CD_PRUNE, A2, WALL_SEC, TELEMETRY, TARGET are not implemented)\n");
     return 2;
   }
   const long long budget = env_ll("WZ_FH_AB_BUDGET", 200000);
@@ -182,7 +183,7 @@ int main(int argc, char **argv) {
   wzfh_problem *P = nullptr;
   int rc = wzfh_init(&cfg, &P);
   if (rc == WZFH_ETRUNC) {
-    printf("RESULT: CELL LIST TRUNCATED at the 20M cap -- refusing to search an incomplete tile\n");
+    printf("RESULT: CELL LIST TRUNCATED at the 20M cap — refusing to search an incomplete tile\n");
     return 2;
   }
   if (rc != WZFH_OK) {
@@ -191,6 +192,14 @@ int main(int argc, char **argv) {
   }
   const int m = cfg.m6 ? 6 : 3;
   const int ncells = wzfh_num_cells(P);
+  long long qps[7] = {0, 0, 0, 0, 0, 0, 0};
+  const bool qp_active = wzfh_qprune_stats(P, qps) == 1;
+  if (cfg.canon && cfg.orbit_qprune && !qp_active && cfg.orbit_q &&
+      (cfg.orbit_q == 2 || cfg.endpos || ((n % 2 == 0) && (((c + d) % 4 + 4) % 4 == 0))))
+    printf("[qprune] DISABLED (needs mod-6 cells and an untruncated cell list)\n");
+  if (qp_active)
+    printf("[qprune] missing_Q_images=%lld not_realizable=%lld eq212=%lld nonintegral=%lld unknown=%lld dead_orbits=%lld"
+           " cells_removed=%lld\n", qps[0], qps[1], qps[2], qps[3], qps[4], qps[5], qps[6]);
   const bool pins_dropped = cfg.canon && (c == 0 || d == 0);
   char digest[64] = {0};
   wzfh_order_digest(P, digest, sizeof digest);
@@ -198,7 +207,7 @@ int main(int argc, char **argv) {
     printf("[order] ord3 cells=%d digest=%s\n", ncells, digest);
     if (const char *e = getenv("WZ_FH_EXPECT_DIGEST"))
       if (strcmp(e, digest) != 0) {
-        printf("RESULT: DIGEST MISMATCH (expected %s) -- refusing to search or resume\n", e);
+        printf("RESULT: DIGEST MISMATCH (expected %s) — refusing to search or resume\n", e);
         return 2;
       }
     if (env_set("WZ_FH_LIST_ONLY")) return 0;
@@ -268,6 +277,7 @@ int main(int argc, char **argv) {
     const bool q_safe = cfg.endpos || ((n % 2 == 0) && (((c + d) % 4 + 4) % 4 == 0));
     if (cfg.orbit_q == 2 || q_safe) cfgsig += ".oq1";
   }
+  if (qp_active) cfgsig += ".qp1";
   if (cfg.order == 3) cfgsig += ".bud" + std::to_string(budget) + ".dg" + (digest[0] ? std::string(digest) : "none");
 
   long long cand = 0, pre_rej = 0, score_rej = 0, clean_no = 0, aborted = 0, cells_done = 0, cells_empty = 0;
@@ -298,7 +308,7 @@ int main(int argc, char **argv) {
         cum_done_base = cdc; cum_dup_base = cdu; cum_dead_base = cde; cum_empty_base = cem;
         printf("[firsthit ckpt] RESUME pi=%lld batch=%lld k=%lld tested_cum=%lld\n", rpi, rb, rk, tc);
       } else {
-        printf("[firsthit ckpt] STALE/MISMATCHED checkpoint IGNORED (%s != %s) -- starting FRESH\n", fsig, cfgsig.c_str());
+        printf("[firsthit ckpt] STALE/MISMATCHED checkpoint IGNORED (%s != %s) — starting FRESH\n", fsig, cfgsig.c_str());
       }
     }
   }
@@ -352,15 +362,20 @@ int main(int argc, char **argv) {
   else if (cell_order && drain_top > 0) stream_cap = drain_top > 0 ? drain_batches * buf_cap : 0;
 
   const int kWindow = (int)std::max(1LL, env_ll("WZFH_PREFETCH_CELLS", 256));
+  // completion slices of the drain (candidates per wzfh_complete call; adapted to slice_sec)
+  const size_t slice_max = (size_t)std::max(1LL, env_ll("WZFH_DRAIN_SLICE_MAX", 4096));
+  size_t slice_len = std::min<size_t>(16, slice_max);
+  double slice_sec = 2.0;
+  if (const char *e = getenv("WZFH_DRAIN_SLICE_SEC")) slice_sec = std::max(0.0, atof(e));
   std::vector<int> win_cells;
   std::vector<wzfh_cell_stats> win_stats;
   std::vector<long long> win_offs;
   std::vector<uint64_t> win_cd;
   size_t win_begin = 0, win_end = 0;   // owned-index range currently streamed
-  auto stream_window = [&](size_t from, long long cap_now) -> int {
+  auto stream_window = [&](size_t from, long long cap_now, int max_cells) -> int {
     win_cells.clear();
     size_t i = from;
-    for (; i < owned.size() && (int)win_cells.size() < kWindow; i++)
+    for (; i < owned.size() && (int)win_cells.size() < max_cells; i++)
       if ((owned[i].kind == 0 || owned[i].kind == 3) && !(resuming && owned[i].pi < res_pi)) win_cells.push_back(owned[i].pi);
     win_begin = from; win_end = i;
     if (win_cells.empty()) return WZFH_OK;
@@ -397,12 +412,17 @@ int main(int argc, char **argv) {
     if (owned[oi].kind == 3) cells_uncap++;
     // streamed window: cells [win_begin, win_end) of `owned`
     long long cap_now = stream_cap;
+    // a cell resumed at drain batch res_batch replays the batches before it and streams one more
+    // buffer even when res_batch is already past the front (the drain that finds the front done
+    // runs when that buffer fills, :2664-2676)
+    const bool resume_cell = resuming && pi == res_pi && cell_order && drain_top > 0 && stream_cap > 0;
+    if (resume_cell) cap_now = std::max(drain_batches, res_batch + 1) * buf_cap;
     if (max_cand > 0) {   // enough to know whether a candidate follows the max_cand-th one
       long long need = max_cand - cand + 1;
       cap_now = cap_now > 0 ? std::min(cap_now, need) : need;
     }
-    if (!(oi >= win_begin && oi < win_end) || max_cand > 0) {
-      int r = stream_window(oi, cap_now);
+    if (!(oi >= win_begin && oi < win_end) || max_cand > 0 || resume_cell) {
+      int r = stream_window(oi, cap_now, resume_cell ? 1 : kWindow);
       if (r != WZFH_OK) { printf("RESULT: DEVICE ERROR (%s)\n", wzfh_last_error()); return 2; }
     }
     int wi = -1;
@@ -438,16 +458,31 @@ int main(int argc, char **argv) {
     std::vector<Buf> cellbuf;
     long long cur_batch = 0, cell_done_ct = 0;
     bool cell_stop = false;
+    // The sorted ranks [start, stop_at) go to the device in slices, so SIGTERM, a hit and the
+    // checkpoint clock are looked at between slices (the reference looks after every
+    // completion, :2686-2697): a slice is sized to take about slice_sec of device time
+    // (doubling from 16 candidates while slices finish faster, halving when one is slower).
     auto complete_batch = [&](const std::vector<Buf> &items, size_t start, size_t stop_at, auto &&on_each) {
-      if (start >= stop_at) return true;
       std::vector<uint64_t> cd;
       std::vector<int> cellof;
-      for (size_t ci = start; ci < stop_at; ci++) { cd.push_back(items[ci].C); cd.push_back(items[ci].D); cellof.push_back(ab_prof ? pi : -1); }
-      std::vector<wzfh_completion> out(stop_at - start);
-      int r = wzfh_complete(P, cd.data(), cellof.data(), (int)(stop_at - start), budget, out.data());
-      if (r != WZFH_OK && r != WZFH_EINTERNAL) { printf("RESULT: DEVICE ERROR (%s)\n", wzfh_last_error()); exit(2); }
-      for (size_t ci = start; ci < stop_at; ci++)
-        if (!on_each(ci, out[ci - start])) return false;
+      std::vector<wzfh_completion> out;
+      for (size_t s0 = start; s0 < stop_at;) {
+        if (g_sigterm || found) return false;
+        const size_t s1 = std::min(stop_at, s0 + slice_len);
+        cd.clear();
+        cellof.clear();
+        for (size_t ci = s0; ci < s1; ci++) { cd.push_back(items[ci].C); cd.push_back(items[ci].D); cellof.push_back(ab_prof ? pi : -1); }
+        out.assign(s1 - s0, wzfh_completion{});
+        const auto t0 = Clock::now();
+        int r = wzfh_complete(P, cd.data(), cellof.data(), (int)(s1 - s0), budget, out.data());
+        if (r != WZFH_OK && r != WZFH_EINTERNAL) { printf("RESULT: DEVICE ERROR (%s)\n", wzfh_last_error()); exit(2); }
+        const double dt = std::chrono::duration<double>(Clock::now() - t0).count();
+        if (dt < 0.5 * slice_sec && s1 - s0 == slice_len) slice_len = std::min<size_t>(slice_len * 2, slice_max);
+        else if (dt > 2 * slice_sec) slice_len = std::max<size_t>(slice_len / 2, 1);
+        for (size_t ci = s0; ci < s1; ci++)
+          if (!on_each(ci, out[ci - s0])) return false;
+        s0 = s1;
+      }
       return true;
     };
     auto complete_one = [&](size_t rank, const Buf &item, const wzfh_completion &o) -> bool {
@@ -456,7 +491,7 @@ int main(int argc, char **argv) {
       if (r == 0 && test_ierr) r = 4;
       if (r == 4) {
         printf("FH_INTERNAL_ERROR: completer FOUND but NPAF != 0\n");
-        printf("RESULT: INTERNAL ERROR at candidate idx=%lld pi=%d -- arm stopped, candidate NOT marked attempted, checkpoint NOT advanced\n", cand, pi);
+        printf("RESULT: INTERNAL ERROR at candidate idx=%lld pi=%d — arm stopped, candidate NOT marked attempted, checkpoint NOT advanced\n", cand, pi);
         fflush(stdout);
         internal = true;
         return false;
@@ -595,9 +630,9 @@ int main(int argc, char **argv) {
          cum_empty_base + cells_empty, cells_capped, drain_top, drain_batches, fh_end, pins_dropped ? 1 : 0,
          tested_base + completed, ck_pi, ck_batch, ck_k);
   if (hit_idx >= 0) printf("RESULT: FOUND at idx=%lld\n", hit_idx);
-  else if (g_sigterm) printf("RESULT: INTERRUPTED (SIGTERM) -- counts are LOWER BOUNDS, stream NOT exhausted\n");
-  else if (fh_end > 0 && !fh_stop) printf("RESULT: RANGE EXHAUSTED -- this arm completed its owned cells [skip,end)\n");
-  else printf("RESULT: NO HIT within probe budget -- NOT a proof of absence\n");
+  else if (g_sigterm) printf("RESULT: INTERRUPTED (SIGTERM) — counts are LOWER BOUNDS, stream NOT exhausted\n");
+  else if (fh_end > 0 && !fh_stop) printf("RESULT: RANGE EXHAUSTED — this arm completed its owned cells [skip,end)\n");
+  else printf("RESULT: NO HIT within probe budget — NOT a proof of absence\n");
   printf("Time: %gs\n", std::chrono::duration<double>(Clock::now() - T0).count());
   fflush(stdout);
   wzfh_destroy(P);

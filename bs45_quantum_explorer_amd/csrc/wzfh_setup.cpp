@@ -422,9 +422,10 @@ int build_problem(wzfh_problem &P) {
     std::sort(cells.begin(), cells.end(), [m](const HostCell &x, const HostCell &y) { return prof_score(x, m) > prof_score(y, m); });
 
   P.kept.assign(cells.size(), 1);
+  P.qp_active = false;
   if (c.canon) {
-    // Orbit canonicalization (wz_match.cpp:1712-1808; the Q-closure prune is not
-    // supported): the kept cell of an orbit is its member with the smallest key.
+    // Orbit canonicalization (wz_match.cpp:1712-1808): the kept cell of an orbit is its member
+    // with the smallest key, unless the Q-closure prune proves the orbit empty.
     bool q_safe = c.endpos || ((n % 2 == 0) && (((c.c + c.d) % 4 + 4) % 4 == 0));
     bool orbit_q = (c.orbit_q == 2) || (c.orbit_q == 1 && q_safe);
     int nvar = orbit_q ? 64 : 32;
@@ -455,6 +456,48 @@ int build_problem(wzfh_problem &P) {
       auto it = orbit_min.find(best);
       if (it == orbit_min.end() || own[i] < it->second) orbit_min[best] = own[i];
     }
+    // Q-closure orbit elimination (WZ_FH_ORBIT_QPRUNE, wz_match.cpp:1735-1800): the list holds
+    // every cell that can hold a solution, and Q maps solutions to solutions, so a cell whose
+    // Q image is missing from the list holds none -- and neither does any member of its orbit --
+    // when the image is certified impossible: not integral, a class sum out of range or of the
+    // wrong parity, or failing the C,D-side eq 2.12 (with Thm 2.12 on). Any other missing image
+    // proves nothing.
+    std::unordered_set<CellKey, CellKeyHash> qp_dead;
+    if (orbit_q && c.orbit_qprune && m == 6) {
+      P.qp_active = true;
+      std::unordered_set<CellKey, CellKeyHash> listed(own.begin(), own.end());
+      int cnt[8];
+      for (int k = 0; k < m; k++) cnt[k] = class_count(n, k, m);
+      auto realizable = [&](const int8_t *p) {
+        for (int k = 0; k < m; k++)
+          if (std::abs(p[k]) > cnt[k] || ((p[k] + cnt[k]) & 1)) return false;
+        return true;
+      };
+      const Eq212 eq212cd(m, n, false);
+      long long *st = P.qp_stats;
+      for (int k = 0; k < 7; k++) st[k] = 0;
+      for (size_t i = 0; i < cells.size(); i++) {
+        const int8_t *x = cells[i].px, *y = cells[i].py;
+        int8_t rx[8], ry[8], qx[8] = {0}, qy[8] = {0};
+        revp(x, rx); revp(y, ry);
+        bool integral = true;
+        for (int k = 0; k < m; k++) {
+          if ((x[k] + y[k] + rx[k] - ry[k]) & 1) integral = false;
+          qx[k] = (int8_t)((x[k] + y[k] + rx[k] - ry[k]) / 2);
+          qy[k] = (int8_t)((x[k] + y[k] - rx[k] + ry[k]) / 2);
+        }
+        if (integral && listed.count(cell_rank_key(qx, qy, m))) continue;
+        st[0]++;
+        if (!integral) st[3]++;
+        else if (!realizable(qx) || !realizable(qy)) st[1]++;
+        else if (c.thm212 && !eq212cd.ok(Tup(qx, qx + m), Tup(qy, qy + m))) st[2]++;
+        else { st[4]++; continue; }
+        qp_dead.insert(oid[i]);
+      }
+      st[5] = (long long)qp_dead.size();
+      for (size_t i = 0; i < cells.size(); i++) st[6] += qp_dead.count(oid[i]);
+    }
+    auto keep_of = [&](size_t i) { return orbit_min[oid[i]] == own[i] && !qp_dead.count(oid[i]); };
     if (c.order == 3) {   // Pass H total order (wz_match.cpp:1816-1856)
       std::unordered_map<CellKey, long long, CellKeyHash> omin;
       for (size_t i = 0; i < cells.size(); i++) {
@@ -480,13 +523,13 @@ int build_problem(wzfh_problem &P) {
       auto fnv = [](uint64_t h, const std::string &k) { for (unsigned char ch : k) { h ^= ch; h *= 1099511628211ULL; } return h; };
       for (size_t i = 0; i < cells.size(); i++) {
         dl = fnv(dl, cell_string(cells[i].px, cells[i].py, m));
-        dk = fnv(dk, orbit_min[oid[i]] == own[i] ? "1" : "0");
+        dk = fnv(dk, keep_of(i) ? "1" : "0");
       }
       char buf[64];
       std::snprintf(buf, sizeof buf, "%016llx:%016llx", (unsigned long long)dl, (unsigned long long)dk);
       P.digest = buf;
     }
-    for (size_t i = 0; i < cells.size(); i++) P.kept[i] = orbit_min[oid[i]] == own[i];
+    for (size_t i = 0; i < cells.size(); i++) P.kept[i] = keep_of(i);
   } else if (c.order == 3) {
     set_error("PROF_ORDER=3 requires WZ_FH_ORBIT_CANON=1");
     return WZFH_EINVAL;

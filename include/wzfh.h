@@ -66,6 +66,9 @@ typedef struct {
   int outer_k;       /* WZ_FH_OUTER_K (0, 2, 3; default 3)                          */
   int no_canon_ab;   /* WZ_FH_NO_CANON                                              */
   long long abp_cap; /* WZ_FH_AB_PROF_CAP (default 20000)                           */
+  int orbit_qprune;  /* WZ_FH_ORBIT_QPRUNE: Q-closure orbit elimination (needs canon,
+                        the quad switch Q active and mod-6 cells; else ignored)        */
+  int reserved;
 } wzfh_config;
 
 typedef struct wzfh_problem wzfh_problem;
@@ -95,6 +98,10 @@ int wzfh_live_cells(const wzfh_problem *p, int *out, int cap);
 /* 64-bit FNV digests "<list>:<kept>" of the ordered cell list under order 3
  * (wz_match.cpp:1834-1856); empty string otherwise. buf needs 34 bytes. */
 int wzfh_order_digest(const wzfh_problem *p, char *buf, int buflen);
+/* Q-closure prune of the cell list (WZ_FH_ORBIT_QPRUNE, wz_match.cpp:1735-1800): stats[0..5] =
+ * missing Q images, not realizable, eq 2.12, nonintegral, unknown, dead orbits; stats[6] = cells
+ * removed. Returns 1 when the prune ran (the arm's CFGSIG gains ".qp1"), 0 when it did not. */
+int wzfh_qprune_stats(const wzfh_problem *p, long long *stats);
 
 /* ---- C,D cell stream (count_pairs22 + hall_ok_pair_fast, wz_match.cpp:678-784,
  * 143-168, on the C,D side of a mod-m cell) ------------------------------------ */
@@ -278,6 +285,13 @@ int wzfh_join_build(wzfh_join *j, int slots_log2, wzfh_join_result *out);
 /* Phase 2 (STREAM + probe, :2970-3010): stream the A,B side against the table. Fills
  * ab_*, hits, found and the solution; *out must hold wzfh_join_build's fields. */
 int wzfh_join_probe(wzfh_join *j, wzfh_join_result *out);
+/* Every solution found by the probe: each A,B hit paired with every C,D of the stream whose
+ * vector it matched (the reference resolves hits by re-enumerating C,D and rechecking NPAF,
+ * :3039-3090, and prints the first; this lists them all -- the full solution set of the
+ * signed signature in the Thm 2.2 space, each with NPAF = 0 at every shift). Writes *count
+ * quadruples (A, B, C, D) of packed words to out in ascending order, or only counts when out
+ * is NULL; WZFH_ECAPACITY (with *count set) when cap is smaller. Call after wzfh_join_probe. */
+int wzfh_join_solutions(wzfh_join *j, uint64_t *out, long long cap, long long *count);
 void wzfh_join_close(wzfh_join *j);
 
 /* ---- Exact checks ----------------------------------------------------------- */
@@ -312,6 +326,10 @@ typedef struct {
   long long launches[2];
 } wzfh_kernel_sums;
 int wzfh_kernel_totals(wzfh_problem *p, wzfh_kernel_sums *out, int reset);
+/* The same totals over a server: its stream launches (which run on per-group handles of the
+ * server's own, several classes of one n side by side) and its problems' completer launches.
+ * No reference counterpart. */
+int wzfh_serve_kernel_totals(wzfh_server *s, wzfh_kernel_sums *out, int reset);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,13 @@ Fixtures:
   joins.json          WZ_JOIN22 runs (single thread): both sides' profile counts, the C,D
                       stream size, distinct keys, raw key hits, FOUND/EXHAUSTED and the
                       printed solution (n = 6..15 every class, n = 17 and 19 samples)
+  resume.json         checkpoint / resume pairs: a run stopped by WZ_FH_TEST_STOP_AFTER, its
+                      checkpoint bytes, an optional forgery of it, the resumed run's lines
+                      and checkpoint bytes (the reference's own resume gates at n = 13, a
+                      stale CFGSIG, n = 19 front-only interrupts)
+  qprune_cells.json   the Q-closure prune (WZ_FH_ORBIT_QPRUNE with the 64-element orbit
+                      group): [qprune] counts, kept orbits, order-3 digests and the per-cell
+                      streams of the cells left (n = 10..14 safe classes, n = 11, 13 ENDPOS)
 
 Usage: python tests/golden/make_golden.py [--only NAME] [--jobs 8]
 """
@@ -343,6 +350,140 @@ def gen_join_signed(jobs):
           f'{sum(r["exhausted"] for r in recs)} exhausted')
 
 
+SEMANTIC = ('RESULT:', 'candidates_streamed=', 'backtracks_entered=', 'A =', 'B =', 'C =', 'D =', 'VERIFY:',
+            'FIRSTHIT:', 'sig =', '[firsthit ckpt]')
+
+
+def semantic(out):
+    """The lines a resume gate compares (tools/test_firsthit_telemetry.py:26-32), plus the
+    checkpoint RESUME / STALE lines; timing fields dropped."""
+    keep = []
+    for line in out.splitlines():
+        if line.startswith(SEMANTIC):
+            keep.append(re.sub(r' elapsed=.*', '', line))
+    return keep
+
+
+def resume_case(name, n, sig, cfg, stop_after, forge=None):
+    """One interrupt/resume pair in a fresh checkpoint directory: a run stopped by
+    WZ_FH_TEST_STOP_AFTER (a SIGTERM after exactly that many completions, wz_match.cpp:2608-2612),
+    its checkpoint bytes, an optional textual forgery of that checkpoint, then the resumed run
+    and its checkpoint bytes. With stop_after = 0 the first run is uninterrupted."""
+    with tempfile.TemporaryDirectory() as td:
+        ck = Path(td)
+        base = {**cfg, 'WZ_FH_CKPT_DIR': ck}
+        first = {**base, 'WZ_FH_TEST_STOP_AFTER': stop_after} if stop_after else base
+        rc1, out1 = run(n, sig, first)
+        p = ck / 'arm_0.ckpt'
+        ck1 = p.read_text() if p.exists() else None
+        forged = None
+        if forge and ck1 is not None:
+            forged = ck1
+            for pat, rep in forge:
+                forged = re.sub(pat, rep, forged)
+            p.write_text(forged)
+        rc2, out2 = run(n, sig, base)
+        ck2 = p.read_text() if p.exists() else None
+    return dict(name=name, n=n, sig=list(sig), settings={k: v for k, v in cfg.items()}, stop_after=stop_after,
+                forge=forge, first=dict(rc=rc1, lines=semantic(out1), ckpt=ck1), forged=forged,
+                resumed=dict(rc=rc2, lines=semantic(out2), ckpt=ck2))
+
+
+def gen_resume(jobs):
+    """Checkpoint / resume fixtures: the reference's own resume gates at n = 13
+    (tools/test_firsthit_telemetry.py:129-171 mid-drain interrupts and buffer-boundary forgeries,
+    tools/test_resume_boundary.py:26-56 front-only forgeries at batch == DRAIN_BATCHES), a stale
+    CFGSIG, and n = 19 front-only interrupts with the default completer."""
+    sig13 = (2, 4, 3, 5)
+    common = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_FH_AB_PROF=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_ORBIT_CANON=1,
+                  WZ_FH_PROF_ORDER=1, WZ_FH_BUF_CAP=16, WZ_FH_DRAIN_TOP=5, WZ_FH_MAX_CAND=256,
+                  WZ_FH_PROG_SEC=9999, WZ_FH_OUTER_K=0)
+    tasks = []
+    for rev, buffered, stop in ((0, 1, 3), (1, 1, 3), (0, 0, 3), (1, 0, 7), (0, 1, 7), (1, 1, 10)):
+        cfg = {**common, 'WZ_FH_AB_BUDGET': 1, 'WZ_FH_STREAM_REV': rev, 'WZ_FH_CELL_ORDER': buffered}
+        tasks.append((f'interrupt rev={rev} buffered={buffered} stop={stop}', 13, sig13, cfg, stop, None))
+    for boundary in (0, 1):
+        cfg = {**common, 'WZ_FH_AB_BUDGET': 1, 'WZ_FH_BUF_CAP': 2, 'WZ_FH_DRAIN_TOP': 0}
+        tasks.append((f'buffer boundary batch={boundary}', 13, sig13, cfg, 2,
+                      [(r'resume_batch=0\nresume_k=2\n', f'resume_batch={boundary}\nresume_k=0\n')]))
+    front = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_FH_AB_PROF=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_ORBIT_CANON=1,
+                 WZ_FH_PROF_ORDER=1, WZ_FH_AB_BUDGET=1, WZ_FH_PROG_SEC=9999)
+    for buf, top, batches in ((4, 2, 1), (4, 2, 2), (6, 3, 1), (6, 3, 3), (8, 5, 2)):
+        cfg = {**front, 'WZ_FH_BUF_CAP': buf, 'WZ_FH_DRAIN_TOP': top, 'WZ_FH_DRAIN_BATCHES': batches}
+        tasks.append((f'front boundary buf={buf} top={top} batches={batches}', 13, sig13, cfg, 1,
+                      [(r'resume_batch=\d+', f'resume_batch={batches}'), (r'resume_k=\d+', 'resume_k=0')]))
+        tasks.append((f'front uninterrupted buf={buf} top={top} batches={batches}', 13, sig13, cfg, 0, None))
+    cfg = {**common, 'WZ_FH_AB_BUDGET': 30}
+    tasks.append(('stale cfgsig', 13, sig13, cfg, 4, [(r'CFGSIG=n13\.', 'CFGSIG=n13.x')]))
+    tasks.append(('resume past a hit', 13, sig13, {**common, 'WZ_FH_AB_BUDGET': 0}, 0, None))
+    for sig in classes(19)[:2]:
+        cfg = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_ORBIT_CANON=1, WZ_FH_PROF_ORDER=1,
+                   WZ_FH_AB_BUDGET=12, WZ_FH_BUF_CAP=64, WZ_FH_DRAIN_TOP=8, WZ_FH_MAX_CAND=3000,
+                   WZ_FH_PROG_SEC=9999)
+        for stop in (5, 37):
+            tasks.append((f'n19 front stop={stop}', 19, sig, cfg, stop, None))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: resume_case(*t), tasks))
+    (HERE / 'resume.json').write_text(json.dumps(recs, indent=0))
+    print(f'resume.json: {len(recs)} interrupt/resume pairs, '
+          f'{sum(1 for r in recs if "RESUME" in " ".join(r["resumed"]["lines"]))} resumed from a checkpoint')
+
+
+QPRUNE = re.compile(r'^\[qprune\] missing_Q_images=(\d+) not_realizable=(\d+) eq212=(\d+) nonintegral=(\d+) '
+                    r'unknown=(\d+) dead_orbits=(\d+) cells_removed=(\d+)', re.M)
+ORD3 = re.compile(r'^\[order\] ord3 cells=(\d+) kept=(\d+) .*digest=(\S+)', re.M)
+
+
+def qprune_case(n, sig, endpos, order):
+    """The Q-closure prune (WZ_FH_ORBIT_QPRUNE=1 with the 64-element group): the reference's
+    [qprune] counts, the order-3 digest when order = 3, and the per-cell stream of every cell
+    the arm streams (CELLSIZE/CDSTAT counts and DUMP SHA-1, as stream_cells.json)."""
+    cfg = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_PROF_ORDER=order,
+               WZ_FH_ORBIT_CANON=1, WZ_FH_ORBIT_Q=1, WZ_FH_ORBIT_QPRUNE=1, WZ_FH_PROG_SEC=99999,
+               WZ_FH_CELLSIZE=10 ** 12)
+    if endpos:
+        cfg['WZ_FH_ENDPOS'] = 1
+    with tempfile.TemporaryDirectory() as td:
+        dump = Path(td) / 'dump.txt'
+        rc, out = run(n, sig, {**cfg, 'WZ_FH_DUMP': dump})
+        lines = dump.read_text().splitlines(keepends=True)
+    assert rc in (0, 3), (n, sig, rc, out[-400:])
+    q = QPRUNE.search(out)
+    assert q, out[-600:]
+    cells = [tuple(int(g) for g in m.groups()) for m in CELL.finditer(out)]
+    nodes = {int(m[1]): int(m[2]) for m in CDSTAT.finditer(out)}
+    rows, pos = [], 0
+    for pi, cand, capped, partial, leaves, hok in cells:
+        h = hashlib.sha1(''.join(lines[pos:pos + cand]).encode()).hexdigest()
+        pos += cand
+        rows.append([pi, cand, leaves, hok, nodes[pi], h])
+    m = re.search(r'\[firsthit\] profiles=(\d+)', out)
+    kept = re.search(r'kept_orbits=(\d+)', out)
+    o3 = ORD3.search(out)
+    return dict(n=n, sig=list(sig), endpos=endpos, order=order, profiles=int(m[1]), kept=int(kept[1]),
+                qprune=[int(g) for g in q.groups()], digest=o3[3] if o3 else None, cells=rows)
+
+
+def gen_qprune(jobs):
+    tasks = []
+    for n in (10, 12, 14):
+        for sig in classes(n):
+            if (sig[2] + sig[3]) % 4 == 0:
+                tasks.append((n, sig, 0, 1))
+    for n in (11, 13):
+        for sig in classes(n):
+            tasks.append((n, sig, 1, 1))
+    for n in (12, 16):
+        for sig in classes(n)[:3]:
+            if (sig[2] + sig[3]) % 4 == 0:
+                tasks.append((n, sig, 0, 3))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: qprune_case(*t), tasks))
+    (HERE / 'qprune_cells.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'qprune_cells.json: {len(recs)} runs, {sum(len(r["cells"]) for r in recs)} streamed cells, '
+          f'{sum(r["qprune"][6] for r in recs)} cells removed by the prune')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--only', default='')
@@ -351,7 +492,8 @@ def main():
     if not REF.exists():
         sys.exit(f'{REF} missing: run make -C oracle ref (needs the reference tree)')
     gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29,
-                solutions=gen_solutions, join=gen_join, join_signed=gen_join_signed)
+                solutions=gen_solutions, join=gen_join, join_signed=gen_join_signed, resume=gen_resume,
+                qprune=gen_qprune)
     for name, g in gens.items():
         if not a.only or name in a.only.split(','):
             g(a.jobs)

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import bs45_quantum_explorer_amd as wz
-from golden_data import complete_cases, stream_runs
+from golden_data import complete_cases, load, stream_runs
 
 pytestmark = pytest.mark.gpu
 
@@ -255,3 +255,20 @@ def test_matrix_core_leaf_test_agrees_with_fp64_on_every_leaf(n, sig, monkeypatc
         assert P.last_kernel(10)[0] > 0   # the margin in use
     finally:
         P.close()
+
+
+QPRUNE = load('qprune_cells.json')
+
+
+@pytest.mark.parametrize('n', sorted({r['n'] for r in QPRUNE}))
+def test_stream_after_qprune_matches_reference(n):
+    """The cells the Q-closure prune leaves (WZ_FH_ORBIT_QPRUNE with the 64-element group) stream
+    the reference's candidates, leaves, hall_ok and DFS nodes, byte for byte."""
+    for rec in (r for r in QPRUNE if r['n'] == n):
+        P = wz.Problem(n, rec['sig'], m6=1, thm211b=1, thm212=1, canon=1, orbit_q=1, orbit_qprune=1,
+                       endpos=rec['endpos'], order=rec['order'], rev=0)
+        try:
+            assert P.live_cells() == [c[0] for c in rec['cells']]
+            assert _cells_of(dict(rec, rev=0), P) == rec['cells'], (n, rec['sig'])
+        finally:
+            P.close()

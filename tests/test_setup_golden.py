@@ -57,3 +57,38 @@ def test_order3_list_digest_matches_reference(n):
         kept = sum(P.cell(pi)[2] for pi in range(P.num_cells()))
         assert (P.num_cells(), kept, P.digest()) == (r['cells'], r['kept'], r['digest']), (n, r['sig'], r['orbit_q'])
         P.close()
+
+
+QPRUNE = load('qprune_cells.json')
+
+
+@pytest.mark.parametrize('rec', QPRUNE, ids=[f"{r['n']}-{tuple(r['sig'])}-ep{r['endpos']}-o{r['order']}" for r in QPRUNE])
+def test_qprune_cell_list_matches_reference(rec):
+    """WZ_FH_ORBIT_QPRUNE (wz_match.cpp:1735-1800): the orbits the Q-closure prune proves empty, the
+    kept orbits left, the order-3 digest over them and the cells the arm streams equal the
+    reference's ([qprune] counts, kept_orbits, [order] digest, CELLSIZE cells)."""
+    P = wz.Problem(rec['n'], rec['sig'], device=False, m6=1, thm211b=1, thm212=1, canon=1, orbit_q=1,
+                   orbit_qprune=1, endpos=rec['endpos'], order=rec['order'])
+    try:
+        ran, st = P.qprune_stats()
+        assert ran and st == rec['qprune']
+        assert P.num_cells() == rec['profiles']
+        assert sum(P.cell(pi)[2] for pi in range(P.num_cells())) == rec['kept']
+        assert P.live_cells() == [c[0] for c in rec['cells']]
+        if rec['digest']:
+            assert P.digest() == rec['digest']
+    finally:
+        P.close()
+
+
+def test_qprune_needs_the_quad_switch():
+    """Without Q active (n odd and no ENDPOS) the prune does not run, and the kept set is the
+    plain orbit canon's."""
+    a = wz.Problem(13, (0, 2, 1, 7), device=False, m6=1, thm211b=1, thm212=1, canon=1, orbit_q=1, orbit_qprune=1)
+    b = wz.Problem(13, (0, 2, 1, 7), device=False, m6=1, thm211b=1, thm212=1, canon=1, orbit_q=1)
+    try:
+        assert a.qprune_stats()[0] is False
+        assert a.live_cells() == b.live_cells()
+    finally:
+        a.close()
+        b.close()
