@@ -230,8 +230,7 @@ def main():
             got, dt = served_pass(server, reqs)
             served.update({rq: r.result for rq, r in zip(reqs, got)})
             log(f'served warmup {w + 1}/{a.warmup} (slice {w % a.slices}): {len(reqs)} cells, {dt * 1e3:.1f} ms')
-        for P in probs:
-            P.kernel_totals(reset=True)
+        server.kernel_totals(reset=True)
         times, ncells, res_timed = [], 0, []
         t_timed = time.perf_counter()
         for s in range(a.steps):
@@ -243,7 +242,7 @@ def main():
             served.update({rq: r.result for rq, r in zip(reqs, got)})
             log(f'served step {s + 1}/{a.steps} (slice {s % a.slices}): {len(reqs)} cells, {dt * 1e3:.1f} ms')
         timed_wall = time.perf_counter() - t_timed
-        tot = [P.kernel_totals() for P in probs]
+        tot = server.kernel_totals()
         step_s = float(np.mean(times))
         served_rate = ncells / float(np.sum(times))
         # ---- open-loop latency at a fraction of the served rate, over the whole set
@@ -288,7 +287,7 @@ def main():
     hits = sum(r.hit for r in res_timed)
 
     traffic = load_traffic()
-    sums = [[sum(t[k][j] for t in tot) for j in range(3)] for k in range(2)]
+    sums = [list(tot[k]) for k in range(2)]
     roof = {
         'stream': kernel_roofline('wzfh::stream_kernel', OPS_PER_CD_NODE,
                                   f'{OPS_PER_CD_NODE} INT32 ops per counted C,D DFS node (rec() entry)',

@@ -164,6 +164,41 @@ __global__ void join_probe(Table t, const uint64_t *abc, unsigned long long N, i
   }
 }
 
+// Member lists: every C,D candidate of the build is linked into the list of its table slot
+// (the slot of its exact vector), so a key's C,D are found without re-streaming.
+__global__ void join_chain(const unsigned long long *slot_of, unsigned long long N, unsigned long long *head,
+                           unsigned long long *next) {
+  const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  next[i] = atomicExch(&head[slot_of[i]], i);
+}
+
+// Solutions of each A,B hit: every C,D on its slot's list. Pass 1 counts, pass 2 writes
+// (A, B, C, D) from the hit's offset.
+__global__ void join_sol_count(const Hit *hits, unsigned long long nh, const unsigned long long *head,
+                               const unsigned long long *next, unsigned long long *cnt) {
+  const unsigned long long h = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  unsigned long long c = 0;
+  for (unsigned long long j = head[hits[h].slot]; j != ~0ull; j = next[j]) c++;
+  cnt[h] = c;
+}
+
+__global__ void join_sol_emit(const Hit *hits, unsigned long long nh, const unsigned long long *head,
+                              const unsigned long long *next, const unsigned long long *offs, const uint64_t *abxy,
+                              const uint64_t *cdxy, uint64_t *out) {
+  const unsigned long long h = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  unsigned long long o = offs[h];
+  const uint64_t A = abxy[2 * hits[h].ab], B = abxy[2 * hits[h].ab + 1];
+  for (unsigned long long j = head[hits[h].slot]; j != ~0ull; j = next[j], o++) {
+    out[4 * o] = A;
+    out[4 * o + 1] = B;
+    out[4 * o + 2] = cdxy[2 * j];
+    out[4 * o + 3] = cdxy[2 * j + 1];
+  }
+}
+
 template <typename T> int dalloc(T **p, size_t count) {
   *p = nullptr;
   if (hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
@@ -218,7 +253,15 @@ struct wzfh_join {
   Table t{};
   unsigned long long slots = 0;
   unsigned long long *ctr = nullptr;   // [0] owners, [1] reprobe, [2] hits
-  bool built = false;
+  bool built = false, probed = false;
+  // kept for wzfh_join_solutions: the C,D candidates and their slots (build), the A,B
+  // candidates and the hits (probe), the slots' member lists (built on first use)
+  uint64_t *cdxy = nullptr, *abxy = nullptr;
+  unsigned long long *cd_slot = nullptr, *head = nullptr, *next = nullptr;
+  Hit *hits = nullptr;
+  unsigned long long ncd = 0, nhits = 0;
+  void drop_build() { dfree(cdxy); dfree(cd_slot); dfree(head); dfree(next); ncd = 0; }
+  void drop_probe() { dfree(abxy); dfree(hits); nhits = 0; probed = false; }
 };
 
 extern "C" {
@@ -271,15 +314,23 @@ int wzfh_join_build(wzfh_join *J, int slots_log2, wzfh_join_result *r) {
   t.vs = vs;
   t.n = J->n;
   J->slots = slots;
+  J->drop_build();
+  J->drop_probe();
+  J->built = false;
   uint64_t *dxy = nullptr;
   unsigned long long *dord = nullptr, *slot_of = nullptr;
   uint8_t *kind = nullptr;
-  auto cleanup = [&]() { dfree(dxy); dfree(dord); dfree(slot_of); dfree(kind); };
+  auto cleanup = [&]() { dfree(dord); dfree(kind); };
   if ((rc = dalloc(&dxy, 2 * (size_t)N)) || (rc = dalloc(&dord, (size_t)N)) || (rc = dalloc(&slot_of, (size_t)N)) ||
       (rc = dalloc(&kind, (size_t)N))) {
     cleanup();
+    dfree(dxy);
+    dfree(slot_of);
     return rc;
   }
+  J->cdxy = dxy;
+  J->cd_slot = slot_of;
+  J->ncd = (unsigned long long)N;
   hipStream_t s = J->stream;
   bool ok = hipMemsetAsync(t.tag, 0, slots * 8, s) == hipSuccess &&
             hipMemsetAsync(J->ctr, 0, 32, s) == hipSuccess &&
@@ -330,12 +381,16 @@ int wzfh_join_probe(wzfh_join *J, wzfh_join_result *r) {
   r->ab_stream = N;
   r->hits = 0;
   r->found = 0;
-  if (N == 0) return WZFH_OK;
+  J->drop_probe();
+  if (N == 0) { J->probed = true; return WZFH_OK; }
   hipStream_t s = J->stream;
+  J->drop_probe();
   uint64_t *dxy = nullptr;
   Hit *hits = nullptr;
   const unsigned long long cap = (unsigned long long)N;
   if ((rc = dalloc(&dxy, 2 * (size_t)N)) || (rc = dalloc(&hits, (size_t)cap))) { dfree(dxy); dfree(hits); return rc; }
+  J->abxy = dxy;
+  J->hits = hits;
   unsigned long long nh = 0;
   bool ok = hipMemsetAsync(J->ctr + 2, 0, 8, s) == hipSuccess &&
             hipMemcpyAsync(dxy, xy.data(), xy.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess;
@@ -348,9 +403,9 @@ int wzfh_join_probe(wzfh_join *J, wzfh_join_result *r) {
   }
   std::vector<Hit> hh(std::min(nh, cap));
   ok = ok && (hh.empty() || hipMemcpy(hh.data(), hits, hh.size() * sizeof(Hit), hipMemcpyDeviceToHost) == hipSuccess);
-  dfree(dxy);
-  dfree(hits);
-  if (!ok) { set_error("join probe: device call failed"); return WZFH_EDEVICE; }
+  if (!ok) { J->drop_probe(); set_error("join probe: device call failed"); return WZFH_EDEVICE; }
+  J->nhits = std::min(nh, cap);
+  J->probed = true;
   r->hits = (long long)nh;
   if (hh.empty()) return WZFH_OK;
   // The reported solution is the one the reference's single-threaded resolve prints: it
@@ -382,8 +437,65 @@ int wzfh_join_probe(wzfh_join *J, wzfh_join_result *r) {
   return WZFH_OK;
 }
 
+int wzfh_join_solutions(wzfh_join *J, uint64_t *out, long long cap, long long *count) {
+  if (!J || !count || cap < 0) return WZFH_EINVAL;
+  *count = 0;
+  if (!J->probed) { set_error("join solutions before probe"); return WZFH_EINVAL; }
+  if (J->nhits == 0) return WZFH_OK;
+  hipStream_t s = J->stream;
+  int rc;
+  if (!J->head) {
+    if ((rc = dalloc(&J->head, (size_t)J->slots)) || (rc = dalloc(&J->next, (size_t)std::max(1ull, J->ncd)))) {
+      dfree(J->head); dfree(J->next);
+      return rc;
+    }
+    bool ok = hipMemsetAsync(J->head, 0xFF, J->slots * 8, s) == hipSuccess;
+    if (ok && J->ncd) {
+      hipLaunchKernelGGL(join_chain, dim3(blocks_for(J->ncd)), dim3(256), 0, s, J->cd_slot, J->ncd, J->head, J->next);
+      ok = hipGetLastError() == hipSuccess;
+    }
+    if (!ok) { dfree(J->head); dfree(J->next); set_error("join solutions: device call failed"); return WZFH_EDEVICE; }
+  }
+  const unsigned long long nh = J->nhits;
+  unsigned long long *cnt = nullptr;
+  if ((rc = dalloc(&cnt, (size_t)nh))) return rc;
+  std::vector<unsigned long long> offs(nh);
+  hipLaunchKernelGGL(join_sol_count, dim3(blocks_for(nh)), dim3(256), 0, s, J->hits, nh, J->head, J->next, cnt);
+  bool ok = hipGetLastError() == hipSuccess &&
+            hipMemcpyAsync(offs.data(), cnt, nh * 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
+            hipStreamSynchronize(s) == hipSuccess;
+  if (!ok) { dfree(cnt); set_error("join solutions: device call failed"); return WZFH_EDEVICE; }
+  unsigned long long total = 0;
+  for (unsigned long long h = 0; h < nh; h++) { const unsigned long long c = offs[h]; offs[h] = total; total += c; }
+  *count = (long long)total;
+  if (!out) { dfree(cnt); return WZFH_OK; }
+  if ((long long)total > cap) { dfree(cnt); set_error("join solutions: output buffer too small"); return WZFH_ECAPACITY; }
+  uint64_t *dout = nullptr;
+  if ((rc = dalloc(&dout, 4 * (size_t)total))) { dfree(cnt); return rc; }
+  ok = hipMemcpyAsync(cnt, offs.data(), nh * 8, hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(join_sol_emit, dim3(blocks_for(nh)), dim3(256), 0, s, J->hits, nh, J->head, J->next, cnt,
+                       J->abxy, J->cdxy, dout);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(out, dout, 32 * (size_t)total, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess;
+  }
+  dfree(dout);
+  dfree(cnt);
+  if (!ok) { set_error("join solutions: device call failed"); return WZFH_EDEVICE; }
+  // ascending (A, B, C, D): the order does not depend on how the device scheduled the lists
+  struct Q { uint64_t w[4]; };
+  Q *q = (Q *)out;
+  std::sort(q, q + total, [](const Q &x, const Q &y) {
+    return std::lexicographical_compare(x.w, x.w + 4, y.w, y.w + 4);
+  });
+  return WZFH_OK;
+}
+
 void wzfh_join_close(wzfh_join *J) {
   if (!J) return;
+  J->drop_build();
+  J->drop_probe();
   dfree(J->t.tag); dfree(J->t.order); dfree(J->t.cd); dfree(J->t.vec);
   dfree(J->ctr);
   if (J->stream) (void)hipStreamDestroy(J->stream);

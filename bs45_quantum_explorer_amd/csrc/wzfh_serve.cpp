@@ -267,6 +267,26 @@ int wzfh_serve_poll(wzfh_server *S, wzfh_serve_result *out, int max, int min_res
   return k;
 }
 
+int wzfh_serve_kernel_totals(wzfh_server *S, wzfh_kernel_sums *out, int reset) {
+  if (!S) return WZFH_EINVAL;
+  wzfh_kernel_sums sum{};
+  // stream launches run on the groups' own handles, completer launches on the problems'
+  std::vector<wzfh_problem *> hs;
+  for (Group &G : S->groups) hs.push_back(G.sh);
+  hs.insert(hs.end(), S->problems.begin(), S->problems.end());
+  for (wzfh_problem *h : hs) {
+    wzfh_kernel_sums t{};
+    if (int rc = wzfh_kernel_totals(h, &t, reset)) return rc;
+    for (int k = 0; k < 2; k++) {
+      sum.ms[k] += t.ms[k];
+      sum.units[k] += t.units[k];
+      sum.launches[k] += t.launches[k];
+    }
+  }
+  if (out) *out = sum;
+  return WZFH_OK;
+}
+
 int wzfh_serve_close(wzfh_server *S) {
   if (!S) return WZFH_EINVAL;
   {

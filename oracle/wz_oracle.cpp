@@ -9,10 +9,13 @@
 #include "wz_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -687,6 +690,56 @@ struct Completer {
   }
 };
 
+// ---- Every solution in the space WZ_JOIN22 searches (wz_match.cpp:2894-2904): both sides
+// enumerated straight from the Thm 2.2 pair encoding (:525-530: quads (X[d], Y[d], X[L-1-d],
+// Y[L-1-d]) with product +1 at depth >= 1; product -1 at depth 0 on the A,B side, any on the
+// C,D side; a free odd middle), with X[0] = +1 pinned when X's sum is 0 (:1526-1527) and the
+// exact signed sums of the signature, then joined on exact autocorrelation vectors. Unlike
+// the reference it uses no profile list and no hall test: both are necessary conditions for a
+// solution, so the set is the same, and this enumeration checks that independently.
+struct SideEnum {
+  int L, half, tx, ty;
+  bool ab, pinx, piny;
+  std::function<void(uint64_t, uint64_t)> sink;   // (X, Y) packed: bit i set = element i is -1
+  void rec(int d, uint64_t X, uint64_t Y, int sx, int sy, int root = -1) {
+    const int rem = L - 2 * d;
+    if (std::abs(tx - sx) > rem || std::abs(ty - sy) > rem) return;
+    if (d == half) {
+      if (L & 1) {
+        for (int t = 0; t < 4; t++) {
+          const int x = (t & 2) ? -1 : 1, y = (t & 1) ? -1 : 1;
+          if (sx + x == tx && sy + y == ty) sink(X | (x < 0 ? 1ull << d : 0), Y | (y < 0 ? 1ull << d : 0));
+        }
+      } else if (sx == tx && sy == ty) {
+        sink(X, Y);
+      }
+      return;
+    }
+    const int i1 = d, i2 = L - 1 - d;
+    for (int q = 0; q < 16; q++) {
+      if (d == 0 && root >= 0 && q != root) continue;
+      const int x1 = (q & 8) ? 1 : -1, y1 = (q & 4) ? 1 : -1, x2 = (q & 2) ? 1 : -1, y2 = (q & 1) ? 1 : -1;
+      const int prod = x1 * y1 * x2 * y2;
+      if (d == 0) {
+        if (ab && prod != -1) continue;
+        if ((pinx && x1 != 1) || (piny && y1 != 1)) continue;
+      } else if (prod != 1) {
+        continue;
+      }
+      const uint64_t nX = X | (x1 < 0 ? 1ull << i1 : 0) | (x2 < 0 ? 1ull << i2 : 0);
+      const uint64_t nY = Y | (y1 < 0 ? 1ull << i1 : 0) | (y2 < 0 ? 1ull << i2 : 0);
+      rec(d + 1, nX, nY, sx + x1 + x2, sy + y1 + y2);
+    }
+  }
+};
+
+// N_X(s) of a packed sequence of length len: (len - s) - 2 * (number of sign changes at
+// distance s), i.e. the sum of x_i x_{i+s} with x_i x_{i+s} = 1 - 2 (b_i xor b_{i+s}).
+inline int npaf_packed(uint64_t x, int len, int s) {
+  if (s >= len) return 0;
+  return (len - s) - 2 * __builtin_popcountll((x ^ (x >> s)) & ((1ull << (len - s)) - 1));
+}
+
 }  // namespace
 
 extern "C" {
@@ -844,6 +897,52 @@ long long wzo_cell_search(void *h, int pi, long long buf_cap, long long node_bud
     }
   }
   return got;
+}
+
+long long wzo_solutions(int n, int a, int b, int c, int d, uint64_t *out, long long cap) {
+  // C,D side: every entry keyed by -(N_C(s) + N_D(s)), s = 1..n (a 64-bit hash indexes exact
+  // vectors); an A,B pair with N_A(s) + N_B(s) equal to it has NPAF = 0 at every shift
+  std::vector<std::pair<uint64_t, uint64_t>> cds;
+  std::vector<int16_t> cdvec;
+  std::unordered_map<uint64_t, std::vector<uint32_t>> byhash;
+  auto vec_hash = [n](const int16_t *v) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int s = 0; s < n; s++) { h ^= (uint16_t)v[s]; h *= 0x100000001b3ull; }
+    return h;
+  };
+  SideEnum cdside{n, n / 2, c, d, false, c == 0, d == 0, {}};
+  cdside.sink = [&](uint64_t X, uint64_t Y) {
+    const size_t at = cdvec.size();
+    for (int s = 1; s <= n; s++) cdvec.push_back((int16_t)-(npaf_packed(X, n, s) + npaf_packed(Y, n, s)));
+    byhash[vec_hash(&cdvec[at])].push_back((uint32_t)cds.size());
+    cds.push_back({X, Y});
+  };
+  cdside.rec(0, 0, 0, 0, 0);
+  // A,B side: one thread per admissible root quad, each probing at its leaves
+  std::vector<std::vector<std::array<uint64_t, 4>>> part(16);
+  std::vector<std::thread> th;
+  for (int root = 0; root < 16; root++) {
+    th.emplace_back([&, root]() {
+      SideEnum abside{n + 1, (n + 1) / 2, a, b, true, a == 0, b == 0, {}};
+      int16_t v[64];
+      abside.sink = [&](uint64_t X, uint64_t Y) {
+        for (int s = 1; s <= n; s++) v[s - 1] = (int16_t)(npaf_packed(X, n + 1, s) + npaf_packed(Y, n + 1, s));
+        auto it = byhash.find(vec_hash(v));
+        if (it == byhash.end()) return;
+        for (uint32_t j : it->second)
+          if (std::memcmp(&cdvec[(size_t)j * n], v, 2 * (size_t)n) == 0)
+            part[root].push_back({X, Y, cds[j].first, cds[j].second});
+      };
+      abside.rec(0, 0, 0, 0, 0, root);
+    });
+  }
+  for (auto &t : th) t.join();
+  std::vector<std::array<uint64_t, 4>> sol;
+  for (auto &p : part) sol.insert(sol.end(), p.begin(), p.end());
+  std::sort(sol.begin(), sol.end());
+  for (long long i = 0; i < (long long)sol.size() && i < cap; i++)
+    for (int t = 0; t < 4; t++) out[4 * i + t] = sol[i][t];
+  return (long long)sol.size();
 }
 
 void wzo_outer_tables(uint8_t *outer2, uint8_t *outer3) {

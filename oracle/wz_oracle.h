@@ -17,6 +17,8 @@
  *                                          budget aborts) + firsthit_runs.json
  *   - npaf / flat score                  : closed form + 43 banked solutions
  *   - outer (A1) tables                  : brute-force enumeration in tests
+ *   - solution set (wzo_solutions)       : brute force over every quadruple at
+ *                                          n = 6..9 in tests, the banked solutions
  */
 #pragma once
 #include <stdint.h>
@@ -92,6 +94,12 @@ typedef struct {
 long long wzo_cell_search(void *h, int pi, long long buf_cap, long long node_budget, int top_k, long long budget,
                           wzo_search_result *res, int *verdicts, long long *nodes, long long *stream_idx,
                           long long *scores);
+/* Every solution (A, B, C, D) of the signed signature in the space WZ_JOIN22 searches
+ * (wz_match.cpp:2894-2904): the Thm 2.2 pair encoding, X[0] = +1 when X's sum is 0, exact
+ * sums; enumerated per side and joined on exact autocorrelation vectors, with no profile list
+ * and no hall test. Writes up to cap quadruples of packed words (A, B: n+1 elements, C, D: n;
+ * bit i set = element i is -1) in ascending (A, B, C, D) order; returns the count. */
+long long wzo_solutions(int n, int a, int b, int c, int d, uint64_t *out, long long cap);
 /* A1 outer tables (wz_match.cpp:560-598) for test cross-checks. */
 void wzo_outer_tables(uint8_t *outer2 /*16*5*9*/, uint8_t *outer3 /*128*5*9*13*/);
 

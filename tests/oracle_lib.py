@@ -63,6 +63,8 @@ def lib():
         L.wzo_cell_search.restype = C.c_longlong
         L.wzo_cell_search.argtypes = [C.c_void_p, C.c_int, C.c_longlong, C.c_longlong, C.c_int, C.c_longlong,
                                       C.POINTER(SearchResult), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.wzo_solutions.restype = C.c_longlong
+        L.wzo_solutions.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong]
         _lib = L
     return _lib
 
@@ -139,6 +141,16 @@ class OracleProblem:
         r = lib().wzo_complete(self.h, pi, Cc.ctypes.data, Dc.ctypes.data, budget, C.byref(nodes),
                                A.ctypes.data, B.ctypes.data)
         return r, nodes.value, A, B
+
+
+def solutions(n, sig):
+    """Every solution of the signed signature in the space WZ_JOIN22 searches (wzo_solutions), as a
+    sorted list of packed (A, B, C, D) word tuples."""
+    a, b, c, d = sig
+    cnt = lib().wzo_solutions(n, a, b, c, d, None, 0)
+    out = np.zeros((max(cnt, 1), 4), np.uint64)
+    lib().wzo_solutions(n, a, b, c, d, out.ctypes.data, cnt)
+    return [tuple(int(v) for v in row) for row in out[:cnt]]
 
 
 def flat_score(n, C_, D_):
