@@ -168,6 +168,7 @@ def main():
     ap.add_argument('--cpu-stride', type=int, default=2, help='CPU sample: every k-th cell of slice 0 per class')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--join-slots-log2', type=int, default=28, help='hash-join table size (0: skip the join)')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
     ap.add_argument('--tune-cells', type=int, default=0,
                     help='cells of the autotuning sample (0: one class slice, the batch a class launch sees)')
@@ -298,6 +299,9 @@ def main():
     }
     dominant = max(roof, key=lambda k: roof[k]['avg_launch_ms'] * roof[k]['launches_timed'])
 
+    join, jroof = (None, {}) if not a.join_slots_log2 else join_scale(a)
+    roof.update(jroof)
+
     cpu = None if a.no_cpu else cpu_baseline(a, cls, probs, slices[0], served)
 
     line = {
@@ -327,7 +331,7 @@ def main():
         'plain_latency_ms': {'p50': pct(plain_lat, 50), 'p99': pct(plain_lat, 99), 'cells': nplain},
         'hits': hits, 'plain_vs_served_mismatch': mismatch, 'false_rejects': verify.get('false_decisions'),
         'verify': verify, 'setup_s': round(setup_s, 1), 'timed_wall_s': round(timed_wall, 3),
-        'roofline': dict(roof[dominant], dominant=True), 'rooflines': roof,
+        'roofline': dict(roof[dominant], dominant=True), 'rooflines': roof, 'join': join,
         'cpu_baseline': cpu, 'autotune': tune,
     }
     print(json.dumps(line), flush=True)
@@ -396,6 +400,51 @@ def serve_sweep(a):
     print(json.dumps(line), flush=True)
     for P in probs:
         P.close()
+
+
+def join_scale(a):
+    """The n = 44 hash join at the configuration's table size: 2^28 slots (44-int16 key vector, tag,
+    order and C,D per slot: 120 B, 32 GB) filled to load 1/2 with random C,D pairs and probed with as
+    many random A,B pairs, the banked BS(45,44)'s images planted on both sides (their hits and the
+    solution list are checked). HBM-bound: algorithmic bytes per insert = the 120 B entry written +
+    the 24 B of input (pair and order) read + one 8 B tag read; per probe = the 16 B pair read + the
+    expected tag reads of an unsuccessful linear probe at load 1/2, (1 + 1/(1 - 1/2)^2) / 2 = 2.5
+    slots x 8 B. Device time from HIP events around the table kernels."""
+    from bs45_quantum_explorer_amd.workload import join_workload
+    sol = next(r for r in json.loads((ROOT / 'tests' / 'golden' / 'solutions.json').read_text())
+               if r['n'] == 44 and not r['quarantined'])
+    lg = a.join_slots_log2
+    nrand = (1 << (lg - 1)) - 8
+    t0 = time.time()
+    cd, ab, pcd, pab = join_workload(44, (sol['A'], sol['B'], sol['C'], sol['D']), nrand, seed=4428)
+    J = wz.Join(44, tuple(sol['sig']))
+    try:
+        r = J.build_from(cd, slots_log2=lg)
+        keys = r.distinct_keys
+        r = J.probe_from(ab)
+        hits = r.hits
+        sols = J.solutions()
+        bms, pms = J.kernel_ms()
+    finally:
+        J.close()
+    ok = hits == len(pab) and keys == nrand + 1 and sorted(sols) == sorted((x, y, c, d) for x, y in pab for c, d in pcd)
+    ins_b = 120 + 24 + 8
+    prb_b = 16 + 2.5 * 8
+    out = {'slots': 1 << lg, 'entry_bytes': 120, 'table_gb': round((1 << lg) * 120 / 1e9, 2), 'inserts': len(cd),
+           'distinct_keys': keys, 'probes': len(ab), 'hits': hits, 'planted_solutions_found': ok,
+           'build_ms': round(bms, 3), 'probe_ms': round(pms, 3),
+           'inserts_per_s': round(len(cd) / (bms / 1e3), 1) if bms else None,
+           'probes_per_s': round(len(ab) / (pms / 1e3), 1) if pms else None,
+           'wall_s': round(time.time() - t0, 1)}
+    roof = {}
+    for name, nunits, per, ms in (('join_build', len(cd), ins_b, bms), ('join_probe', len(ab), prb_b, pms)):
+        gbs = nunits * per / (ms / 1e3) / 1e9 if ms else 0.0
+        roof[name] = {'bound': 'hbm', 'achieved': round(gbs, 2), 'peak': HBM_PEAK_BPS / 1e9, 'unit': 'GB/s',
+                      'frac': round(gbs * 1e9 / HBM_PEAK_BPS, 5), 'traffic': None,
+                      'per_unit': f'{per:g} algorithmic bytes per {"insert" if name == "join_build" else "probe"}',
+                      'units_per_launch': nunits, 'avg_launch_ms': round(ms, 3)}
+    log(f'join 2^{lg} slots: {out}')
+    return out, roof
 
 
 def verify_sample(a, probs, reqs):

@@ -23,7 +23,8 @@ EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy',
            'wzfh_serve_open', 'wzfh_serve_submit', 'wzfh_serve_poll', 'wzfh_serve_close',
            'wzfh_join_open', 'wzfh_join_build', 'wzfh_join_probe', 'wzfh_join_close',
            'wzfh_autotune', 'wzfh_get_tuning', 'wzfh_set_tuning', 'wzfh_kernel_totals', 'wzfh_complete_noalloc',
-           'wzfh_serve_kernel_totals', 'wzfh_join_solutions', 'wzfh_qprune_stats')
+           'wzfh_serve_kernel_totals', 'wzfh_join_solutions', 'wzfh_qprune_stats', 'wzfh_complete_depth',
+           'wzfh_join_build_from', 'wzfh_join_probe_from', 'wzfh_join_kernel_ms')
 NOALLOC_MAX_CAND = 1024            # WZFH_NOALLOC_MAX_CAND
 NOALLOC_SCRATCH_BYTES = 64 << 20   # WZFH_NOALLOC_SCRATCH_BYTES
 
@@ -117,6 +118,8 @@ def lib():
         L.wzfh_stream_cells.argtypes = [vp, vp, C.c_int, C.POINTER(StreamOpts), vp, vp, C.c_longlong, vp]
         L.wzfh_complete.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
         L.wzfh_complete_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp]
+        L.wzfh_complete_depth.argtypes = [vp, vp, vp, C.c_int, C.c_longlong, vp, vp]
+        L.wzfh_qprune_stats.argtypes = [vp, vp]
         L.wzfh_search_cells.argtypes = [vp, vp, C.c_int, C.POINTER(SearchOpts), vp]
         L.wzfh_npaf_check.argtypes = [C.c_int, vp, vp, vp, vp, C.POINTER(C.c_int)]
         L.wzfh_flat_score.argtypes = [C.c_int, C.c_uint64, C.c_uint64]
@@ -125,6 +128,9 @@ def lib():
         L.wzfh_join_build.argtypes = [vp, C.c_int, vp]
         L.wzfh_join_probe.argtypes = [vp, vp]
         L.wzfh_join_solutions.argtypes = [vp, vp, C.c_longlong, C.POINTER(C.c_longlong)]
+        L.wzfh_join_build_from.argtypes = [vp, vp, C.c_longlong, C.c_int, vp]
+        L.wzfh_join_probe_from.argtypes = [vp, vp, C.c_longlong, vp]
+        L.wzfh_join_kernel_ms.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.wzfh_join_close.argtypes = [vp]
         L.wzfh_join_close.restype = None
         L.wzfh_last_kernel_ms.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
@@ -251,6 +257,24 @@ class Problem:
             raise WzfhError(rc, lib().wzfh_last_error().decode(), got if rc == EINTERNAL else None)
         return got
 
+    def complete_depth(self, cd, budget=0, cell_of=None):
+        """wzfh_complete_depth: the completions and, per candidate, the nodes counted at each depth
+        (a list of (n+1)//2 + 1 ints summing to the candidate's nodes)."""
+        cd = np.ascontiguousarray(np.asarray(cd, dtype=np.uint64).reshape(-1, 2))
+        nc = len(cd)
+        out = (Completion * max(nc, 1))()
+        dep = np.zeros((max(nc, 1), 64), np.int64)
+        co = None
+        if cell_of is not None:
+            co = np.ascontiguousarray(np.asarray(cell_of, dtype=np.int32))
+        rc = lib().wzfh_complete_depth(self.h, cd.ctypes.data, co.ctypes.data if co is not None else None, nc, budget,
+                                       out, dep.ctypes.data)
+        got = [out[i] for i in range(nc)]
+        if rc != OK:
+            raise WzfhError(rc, lib().wzfh_last_error().decode(), got if rc == EINTERNAL else None)
+        k = (self.n + 1) // 2 + 1
+        return got, [list(map(int, dep[i, :k])) for i in range(nc)]
+
     def search(self, cells, top_k, budget, cand_cap=0, node_budget=0, batched=True, split_depth=0, window=0):
         cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int32))
         nc = len(cells)
@@ -356,6 +380,23 @@ class Join:
     def probe(self):
         _check(lib().wzfh_join_probe(self.h, C.byref(self.result)))
         return self.result
+
+    def build_from(self, cd, slots_log2=0):
+        """Table from explicit C,D pairs (packed words, shape (k, 2))."""
+        cd = np.ascontiguousarray(np.asarray(cd, dtype=np.uint64).reshape(-1, 2))
+        _check(lib().wzfh_join_build_from(self.h, cd.ctypes.data, len(cd), slots_log2, C.byref(self.result)))
+        return self.result
+
+    def probe_from(self, ab):
+        """Probe with explicit A,B pairs (packed words of n+1 elements, shape (k, 2))."""
+        ab = np.ascontiguousarray(np.asarray(ab, dtype=np.uint64).reshape(-1, 2))
+        _check(lib().wzfh_join_probe_from(self.h, ab.ctypes.data, len(ab), C.byref(self.result)))
+        return self.result
+
+    def kernel_ms(self):
+        b, p = C.c_double(), C.c_double()
+        _check(lib().wzfh_join_kernel_ms(self.h, C.byref(b), C.byref(p)))
+        return b.value, p.value
 
     def solutions(self):
         """Every solution found by the probe, as a sorted list of packed (A, B, C, D) word tuples."""

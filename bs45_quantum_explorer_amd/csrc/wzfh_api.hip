@@ -105,7 +105,7 @@ struct DeviceState {
   DBuf dq, cell_lock, cell_crossed, cell_wp, cell_pn, cell_pl, cell_pc, cell_open, cell_next, cell_sp;
   uint32_t epoch = 0;              // launch epoch tagging record states
   DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_icharged, c_ebase, c_epath, c_ebud,
-       c_fv, c_fn, c_fA, c_fB, c_fj, c_fcc, c_fcur, c_scratch;
+       c_fv, c_fn, c_fA, c_fB, c_fj, c_fcc, c_fcur, c_scratch, c_depth;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
   // running totals since the last reset: device time of the stream kernel (kind 0) and of the
@@ -349,7 +349,8 @@ void device_free(wzfh_problem &P) {
                   &S->cell_crossed, &S->cell_wp, &S->cell_pn, &S->cell_pl, &S->cell_pc, &S->cell_open,
                   &S->cell_next, &S->cell_sp, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
                   &S->c_B, &S->c_charges, &S->c_icharged, &S->c_ebase, &S->c_epath, &S->c_ebud,
-                  &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_fj, &S->c_fcc, &S->c_fcur, &S->c_scratch})
+                  &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_fj, &S->c_fcc, &S->c_fcur, &S->c_scratch,
+                  &S->c_depth})
     b->release();
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -700,7 +701,8 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
                            const std::vector<CompleteEntry> &ents, int maxrows, long long budget, int ext,
                            long long item_cap, int *v,
                            long long *nd, uint64_t *A, uint64_t *B, int *fj, long long *fcc, long long *fcur,
-                           double *ms, Carve *carve, std::vector<std::vector<ItemRes>> *items) {
+                           double *ms, Carve *carve, std::vector<std::vector<ItemRes>> *items,
+                           std::vector<long long> *depth = nullptr) {
   DeviceState *S = P.dev;
   const int nent = (int)ents.size();
   const int words = std::max(1, (maxrows + 63) / 64);
@@ -771,6 +773,11 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
   a.outA = bf.A; a.outB = bf.B;
   a.charges = bf.charges;
   a.item_charged = bf.icharged;
+  a.depth_out = nullptr;
+  if (depth && ext == 0 && !carve) {   // per-depth node counts of whole walks (FH_TELEM)
+    if (S->c_depth.ensure(nw * 64 * 8)) return WZFH_ENOMEM;
+    a.depth_out = S->c_depth.as<long long>();
+  }
   WZ_HIP(hipEventRecord(S->ev[2], S->stream));
   WZ_HIP(launch_complete(a, S->kab.as<uint8_t>(), blocks, lds, lds ? lds_bytes : 0, S->stream));
   WZ_HIP(hipEventRecord(S->ev[4], S->stream));
@@ -798,6 +805,11 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
     WZ_HIP(hipStreamSynchronize(S->stream));
     return WZFH_OK;
   };
+  if (a.depth_out) {
+    depth->assign(nw * 64, 0);
+    WZ_HIP(hipMemcpyAsync(depth->data(), a.depth_out, nw * 64 * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipStreamSynchronize(S->stream));
+  }
   if (ext == 0 || check) {
     if (int rc = fetch_items()) return rc;
     // the same scan as the device reduce, on the host
@@ -957,7 +969,7 @@ struct Frame {
 }  // namespace
 
 int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
-                 wzfh_completion *out, Carve *carve) {
+                 wzfh_completion *out, Carve *carve, long long *depth_nodes) {
   DeviceState *S = P.dev;
   if (ncand <= 0) return WZFH_OK;
   WZ_HIP(hipSetDevice(S->device));
@@ -977,7 +989,8 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
   const bool limited = budget > 0;
   std::vector<Result> fin(ncand, Result{2, 0, 0, 0, 0, 0, false});
   double ms = 0;
-  const int ext0 = complete_split_ext(P);
+  // per-depth counts need whole walks: one item per candidate
+  const int ext0 = depth_nodes ? 0 : complete_split_ext(P);
   const int max_base = std::min(P.L1 / 2 - 1, 21);   // paths are 3 bits per depth in 64 bits
   const long long exact_item = exact_item_charges();
   const long long cap = ext0 > 0 ? item_charge_cap(P) : 0;
@@ -1062,9 +1075,15 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
         std::vector<long long> nd(ne), cc(ne), cj(ne);
         std::vector<uint64_t> A(ne), B(ne);
         std::vector<std::vector<ItemRes>> items;
+        std::vector<long long> dep;
         int rc = complete_launch(P, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, icap, v.data(),
-                                 nd.data(), A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms, carve, &items);
+                                 nd.data(), A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms, carve, &items,
+                                 depth_nodes ? &dep : nullptr);
         if (rc != WZFH_OK) return rc;
+        if (depth_nodes && !dep.empty())
+          for (int e = 0; e < ne; e++)
+            std::copy(dep.begin() + (size_t)e * 64, dep.begin() + (size_t)e * 64 + 64,
+                      depth_nodes + (size_t)group[c0 + e].cand * 64);
         const bool root_pair = ext > 0 && !P.cfg.no_canon_ab && kind == 1;
         for (int e = 0; e < ne; e++) {
           const Job &x = group[c0 + e];
@@ -1331,6 +1350,14 @@ int wzfh_complete(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int n
   if (!p || (ncand > 0 && (!cd || !out)) || ncand < 0 || budget < 0) return WZFH_EINVAL;
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
   return run_complete(*p, cd, cell_of, ncand, budget, out);
+}
+
+int wzfh_complete_depth(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                        wzfh_completion *out, long long *depth_nodes) {
+  if (!p || (ncand > 0 && (!cd || !out || !depth_nodes)) || ncand < 0 || budget < 0) return WZFH_EINVAL;
+  if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  std::fill(depth_nodes, depth_nodes + (size_t)ncand * 64, 0ll);
+  return run_complete(*p, cd, cell_of, ncand, budget, out, nullptr, depth_nodes);
 }
 
 int wzfh_complete_noalloc(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand, long long budget,

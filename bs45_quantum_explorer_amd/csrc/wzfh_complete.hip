@@ -157,6 +157,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
     }
     int verdict = 2;
     long long total = 0, cur = 0;   // nodes counted; nodes charged to the budget
+    long long dn = 0;               // lane l: nodes counted at depth l (FH_TELEM depth_nodes)
     uint64_t A = 0, B = 0;
     // pre-filter (wz_match.cpp:1305-1308): |T[s]| > 2 (L - s) for some s < L
     const bool prerej = any_lane(valid && (T > 2 * (L - s) || -T > 2 * (L - s)));
@@ -217,6 +218,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
                   if (mid + s < L) delta += sgn(A, mid + s) * av + sgn(B, mid + s) * bv;
                 }
                 total++;
+                dn += (lane == d) ? 1 : 0;
                 bool ok = (abs(sumA + av) == a.absA) && (abs(sumB + bv) == a.absB);
                 if (ok) ok = !any_lane(valid && (dab + delta != T));
                 if (ok && rows) {
@@ -371,6 +373,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
             if (!((cmask >> k) & 1u)) { state = S_EXHAUSTED; continue; }
             if ((path & ((1ll << sh) - 1)) == 0) {
               total++;
+              dn += (lane == d) ? 1 : 0;
               if (limited && ++cur > bud) { state = S_ABORT; continue; }
             }
             if (charged_out(cur)) { state = S_ABORT; continue; }
@@ -384,6 +387,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
             k = __ffs((int)rest) - 1;
             kcur = k;
             total++;
+            dn += (lane == d) ? 1 : 0;
             if (limited && ++cur > bud) { state = S_ABORT; continue; }
             if (charged_out(cur)) { state = S_ABORT; continue; }
             if (a.item_cap > 0 && cur >= a.item_cap) { cap_stop = true; state = S_ABORT; continue; }
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
         for (int l = d - 1; l >= 0 && ext == 0; l--) {
           const int kl = uni(__builtin_amdgcn_readlane(st_k, l));
           const unsigned cm = (unsigned)uni(__builtin_amdgcn_readlane(st_mask, l)) & 0xFFu;
-          if (cm & (0xFFu << (kl + 1)) & 0xFFu) total++;
+          if (cm & (0xFFu << (kl + 1)) & 0xFFu) { total++; dn += (lane == l) ? 1 : 0; }
         }
         verdict = cap_stop ? 5 : 3;
         A = B = 0;
@@ -528,6 +532,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
     a.outA[u] = A;
     a.outB[u] = B;
     a.charges[u] = cur;
+    if (a.depth_out) a.depth_out[u * 64 + lane] = dn;
   }
 }
 

@@ -91,7 +91,7 @@ void select_topk(int n, const StreamOut &so, int ncells, int K, TopK &t);
 void aggregate_completions(const TopK &t, const wzfh_completion *comp, wzfh_search_result *res);
 struct Carve;
 int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
-                 wzfh_completion *out, Carve *carve = nullptr);
+                 wzfh_completion *out, Carve *carve = nullptr, long long *depth_nodes = nullptr);
 // Problems whose stream kernels use identical tables (same n, residue modulus, stream order,
 // ENDPOS, side and pinned roots) can share one stream launch.
 std::string stream_key(const wzfh_problem &P);

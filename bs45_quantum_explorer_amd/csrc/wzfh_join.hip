@@ -254,6 +254,8 @@ struct wzfh_join {
   unsigned long long slots = 0;
   unsigned long long *ctr = nullptr;   // [0] owners, [1] reprobe, [2] hits
   bool built = false, probed = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double build_ms = 0, probe_ms = 0;   // device time of the table kernels (last build / probe)
   // kept for wzfh_join_solutions: the C,D candidates and their slots (build), the A,B
   // candidates and the hits (probe), the slots' member lists (built on first use)
   uint64_t *cdxy = nullptr, *abxy = nullptr;
@@ -280,22 +282,23 @@ int wzfh_join_open(const wzfh_config *cfg, wzfh_join **out) {
     rc = WZFH_EDEVICE;
   }
   if (rc == WZFH_OK) rc = dalloc(&J->ctr, 4);
+  if (rc == WZFH_OK && (hipEventCreate(&J->ev[0]) != hipSuccess || hipEventCreate(&J->ev[1]) != hipSuccess)) {
+    set_error("hipEventCreate failed (join)");
+    rc = WZFH_EDEVICE;
+  }
   if (rc != WZFH_OK) { wzfh_join_close(J); return rc; }
   *out = J;
   return WZFH_OK;
 }
 
-int wzfh_join_build(wzfh_join *J, int slots_log2, wzfh_join_result *r) {
-  if (!J || !r) return WZFH_EINVAL;
-  *r = wzfh_join_result{};
-  r->cd_profiles = (long long)J->cd->cells.size();
-  r->ab_profiles = (long long)J->ab->cells.size();
-  std::vector<uint64_t> xy;
-  std::vector<unsigned long long> ord;
-  long long N = 0;
-  int rc = stream_side(*J->cd, xy, ord, N);
-  if (rc != WZFH_OK) return rc;
-  r->cd_stream = N;
+}  // extern "C"
+
+// The table from N C,D pairs xy (2 words each) with their order keys (the earliest order of a
+// vector keeps the entry): claim / write / verify rounds until no equal-tag meeting probes on,
+// then the earliest duplicate writes its C,D. Device time of the table kernels -> J->build_ms.
+static int build_table(wzfh_join *J, const uint64_t *xy, const unsigned long long *ord, long long N, int slots_log2,
+                       wzfh_join_result *r) {
+  int rc;
   unsigned long long slots = 1024;
   if (slots_log2 > 0) {
     if (slots_log2 > 40) { set_error("slots_log2 too large"); return WZFH_EINVAL; }
@@ -336,24 +339,32 @@ int wzfh_join_build(wzfh_join *J, int slots_log2, wzfh_join_result *r) {
             hipMemsetAsync(J->ctr, 0, 32, s) == hipSuccess &&
             hipMemsetAsync(kind, 0, (size_t)N, s) == hipSuccess &&
             hipMemsetAsync(slot_of, 0xFF, (size_t)N * 8, s) == hipSuccess &&
-            (N == 0 || (hipMemcpyAsync(dxy, xy.data(), xy.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess &&
-                        hipMemcpyAsync(dord, ord.data(), ord.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess));
+            (N == 0 || (hipMemcpyAsync(dxy, xy, (size_t)N * 16, hipMemcpyHostToDevice, s) == hipSuccess &&
+                        hipMemcpyAsync(dord, ord, (size_t)N * 8, hipMemcpyHostToDevice, s) == hipSuccess));
   const unsigned nb = blocks_for((unsigned long long)N);
+  J->build_ms = 0;
+  float ems = 0;
   for (int round = 0; ok && N > 0 && round < 16; round++) {
+    ok = hipEventRecord(J->ev[0], s) == hipSuccess;
     hipLaunchKernelGGL(join_claim, dim3(nb), dim3(256), 0, s, t, dxy, (unsigned long long)N, kind, slot_of, J->ctr);
     hipLaunchKernelGGL(join_write, dim3(nb), dim3(256), 0, s, t, dxy, dord, (unsigned long long)N, kind, slot_of);
     ok = hipMemsetAsync(J->ctr + 1, 0, 8, s) == hipSuccess;
     hipLaunchKernelGGL(join_verify, dim3(nb), dim3(256), 0, s, t, dxy, dord, (unsigned long long)N, kind, slot_of,
                        J->ctr + 1);
+    ok = ok && hipEventRecord(J->ev[1], s) == hipSuccess;
     unsigned long long re = 0;
     ok = ok && hipGetLastError() == hipSuccess &&
          hipMemcpyAsync(&re, J->ctr + 1, 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
-         hipStreamSynchronize(s) == hipSuccess;
+         hipStreamSynchronize(s) == hipSuccess && hipEventElapsedTime(&ems, J->ev[0], J->ev[1]) == hipSuccess;
+    J->build_ms += ems;
     if (re == 0) break;
   }
   if (ok && N > 0) {
+    ok = hipEventRecord(J->ev[0], s) == hipSuccess;
     hipLaunchKernelGGL(join_rep, dim3(nb), dim3(256), 0, s, t, dxy, dord, (unsigned long long)N, kind, slot_of);
-    ok = hipGetLastError() == hipSuccess;
+    ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(J->ev[1], s) == hipSuccess &&
+         hipEventSynchronize(J->ev[1]) == hipSuccess && hipEventElapsedTime(&ems, J->ev[0], J->ev[1]) == hipSuccess;
+    J->build_ms += ems;
   }
   unsigned long long owners = 0;
   std::vector<uint8_t> kh((size_t)N);
@@ -370,14 +381,37 @@ int wzfh_join_build(wzfh_join *J, int slots_log2, wzfh_join_result *r) {
   return WZFH_OK;
 }
 
-int wzfh_join_probe(wzfh_join *J, wzfh_join_result *r) {
+extern "C" {
+
+int wzfh_join_build(wzfh_join *J, int slots_log2, wzfh_join_result *r) {
   if (!J || !r) return WZFH_EINVAL;
-  if (!J->built) { set_error("join probe before build"); return WZFH_EINVAL; }
+  *r = wzfh_join_result{};
+  r->cd_profiles = (long long)J->cd->cells.size();
+  r->ab_profiles = (long long)J->ab->cells.size();
   std::vector<uint64_t> xy;
   std::vector<unsigned long long> ord;
   long long N = 0;
-  int rc = stream_side(*J->ab, xy, ord, N);
+  int rc = stream_side(*J->cd, xy, ord, N);
   if (rc != WZFH_OK) return rc;
+  r->cd_stream = N;
+  return build_table(J, xy.data(), ord.data(), N, slots_log2, r);
+}
+
+int wzfh_join_build_from(wzfh_join *J, const uint64_t *cd, long long ncd, int slots_log2, wzfh_join_result *r) {
+  if (!J || !r || ncd < 0 || (ncd > 0 && !cd)) return WZFH_EINVAL;
+  *r = wzfh_join_result{};
+  r->cd_stream = ncd;
+  std::vector<unsigned long long> ord((size_t)ncd);
+  for (long long i = 0; i < ncd; i++) ord[(size_t)i] = (unsigned long long)i;
+  return build_table(J, cd, ord.data(), ncd, slots_log2, r);
+}
+
+}  // extern "C"
+
+// Probes the table with N A,B pairs xy (order keys ord decide which hit is reported).
+static int probe_table(wzfh_join *J, const uint64_t *xy, const unsigned long long *ord, long long N,
+                       wzfh_join_result *r) {
+  int rc;
   r->ab_stream = N;
   r->hits = 0;
   r->found = 0;
@@ -393,13 +427,17 @@ int wzfh_join_probe(wzfh_join *J, wzfh_join_result *r) {
   J->hits = hits;
   unsigned long long nh = 0;
   bool ok = hipMemsetAsync(J->ctr + 2, 0, 8, s) == hipSuccess &&
-            hipMemcpyAsync(dxy, xy.data(), xy.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess;
+            hipMemcpyAsync(dxy, xy, (size_t)N * 16, hipMemcpyHostToDevice, s) == hipSuccess;
+  J->probe_ms = 0;
   if (ok) {
+    float ems = 0;
+    ok = hipEventRecord(J->ev[0], s) == hipSuccess;
     hipLaunchKernelGGL(join_probe, dim3(blocks_for((unsigned long long)N)), dim3(256), 0, s, J->t, dxy,
                        (unsigned long long)N, J->n + 1, hits, cap, J->ctr + 2);
-    ok = hipGetLastError() == hipSuccess &&
+    ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(J->ev[1], s) == hipSuccess &&
          hipMemcpyAsync(&nh, J->ctr + 2, 8, hipMemcpyDeviceToHost, s) == hipSuccess &&
-         hipStreamSynchronize(s) == hipSuccess;
+         hipStreamSynchronize(s) == hipSuccess && hipEventElapsedTime(&ems, J->ev[0], J->ev[1]) == hipSuccess;
+    J->probe_ms = ems;
   }
   std::vector<Hit> hh(std::min(nh, cap));
   ok = ok && (hh.empty() || hipMemcpy(hh.data(), hits, hh.size() * sizeof(Hit), hipMemcpyDeviceToHost) == hipSuccess);
@@ -434,6 +472,34 @@ int wzfh_join_probe(wzfh_join *J, wzfh_join_result *r) {
   }
   r->found = 1;
   r->A = A; r->B = B; r->C = cdw[0]; r->D = cdw[1];
+  return WZFH_OK;
+}
+
+extern "C" {
+
+int wzfh_join_probe(wzfh_join *J, wzfh_join_result *r) {
+  if (!J || !r) return WZFH_EINVAL;
+  if (!J->built) { set_error("join probe before build"); return WZFH_EINVAL; }
+  std::vector<uint64_t> xy;
+  std::vector<unsigned long long> ord;
+  long long N = 0;
+  int rc = stream_side(*J->ab, xy, ord, N);
+  if (rc != WZFH_OK) return rc;
+  return probe_table(J, xy.data(), ord.data(), N, r);
+}
+
+int wzfh_join_probe_from(wzfh_join *J, const uint64_t *ab, long long nab, wzfh_join_result *r) {
+  if (!J || !r || nab < 0 || (nab > 0 && !ab)) return WZFH_EINVAL;
+  if (!J->built) { set_error("join probe before build"); return WZFH_EINVAL; }
+  std::vector<unsigned long long> ord((size_t)nab);
+  for (long long i = 0; i < nab; i++) ord[(size_t)i] = (unsigned long long)i;
+  return probe_table(J, ab, ord.data(), nab, r);
+}
+
+int wzfh_join_kernel_ms(const wzfh_join *J, double *build_ms, double *probe_ms) {
+  if (!J) return WZFH_EINVAL;
+  if (build_ms) *build_ms = J->build_ms;
+  if (probe_ms) *probe_ms = J->probe_ms;
   return WZFH_OK;
 }
 
@@ -496,6 +562,7 @@ void wzfh_join_close(wzfh_join *J) {
   if (!J) return;
   J->drop_build();
   J->drop_probe();
+  for (auto &e : J->ev) if (e) (void)hipEventDestroy(e);
   dfree(J->t.tag); dfree(J->t.order); dfree(J->t.cd); dfree(J->t.vec);
   dfree(J->ctr);
   if (J->stream) (void)hipStreamDestroy(J->stream);

@@ -129,6 +129,7 @@ struct CompleteArgs {
   int *verdict;
   long long *nodes;
   uint64_t *outA, *outB;
+  long long *depth_out;        // unsplit launches: [item][64] nodes counted at each depth (null: off)
 };
 
 }  // namespace wzfh

@@ -77,6 +77,57 @@ static void print_solution(int n, const std::vector<int> &A, const std::vector<i
   printf("VERIFY: max |NPAF[s]| over s=1..%d = %d%s\n", n, mx, mx == 0 ? "  (NPAF==0 confirmed)" : "  (NONZERO!)");
 }
 
+// WZ_FH_TELEMETRY (wz_match.cpp:901-950): per-arm diagnostics printed as one FH_TELEM v2 record
+// after the summary, never part of the CFGSIG or the checkpoint. stride 1: every completion's
+// depth histogram and every score call timed; stride 64: a deterministic 1/64 sample (the
+// splitmix64 finalizer of the call's ordinal, low 6 bits zero). Completion times are the wall
+// time of each device slice shared out evenly over the candidates it decided.
+struct Telem {
+  int stride = 0;
+  long long cell_ns = 0, score_ns_est = 0, sort_ns = 0, complete_ns = 0;
+  long long resume_replay_ns = 0, resume_replay_score_ns_est = 0, resume_replay_sort_ns = 0, resume_cells = 0;
+  long long score_calls = 0, score_samples = 0, completions = 0;
+  long long hist_candidates = 0, hist_nodes = 0, total_nodes = 0;
+  long long unranked_nodes = 0, unranked_count = 0, unranked_ns = 0;
+  long long cells_live_done = 0, cells_partial = 0;
+  std::vector<long long> depth_nodes;
+  long long rank_nodes[10] = {0}, rank_count[10] = {0}, rank_ns[10] = {0}, rank_aborts[10] = {0};
+  static long long ns_since(Clock::time_point t) {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t).count();
+  }
+  bool selected(uint64_t ordinal) const {
+    uint64_t x = ordinal + 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return ((x ^ (x >> 31)) & (uint64_t)(stride - 1)) == 0;
+  }
+  void emit(int n, long long worker_ns) const {
+    if (!stride) return;
+    printf("FH_TELEM {\"version\":2,\"n\":%d,\"stride\":%d,\"worker_ns\":%lld", n, stride, worker_ns);
+    const std::pair<const char *, long long> f[] = {
+        {"cell_ns", cell_ns}, {"score_ns_est", score_ns_est}, {"sort_ns", sort_ns},
+        {"resume_replay_ns", resume_replay_ns}, {"resume_replay_score_ns_est", resume_replay_score_ns_est},
+        {"resume_replay_sort_ns", resume_replay_sort_ns}, {"resume_cells", resume_cells},
+        {"complete_ns", complete_ns}, {"score_calls", score_calls}, {"score_samples", score_samples},
+        {"completions", completions}, {"hist_candidates", hist_candidates}, {"hist_nodes", hist_nodes},
+        {"total_nodes", total_nodes}, {"unranked_nodes", unranked_nodes}, {"unranked_count", unranked_count},
+        {"unranked_ns", unranked_ns}, {"cells_live_done", cells_live_done}, {"cells_partial", cells_partial}};
+    for (const auto &kv : f) printf(",\"%s\":%lld", kv.first, kv.second);
+    auto arr = [](const char *key, const long long *v, size_t k) {
+      printf(",\"%s\":[", key);
+      for (size_t i = 0; i < k; i++) printf("%s%lld", i ? "," : "", v[i]);
+      printf("]");
+    };
+    arr("depth_nodes", depth_nodes.data(), depth_nodes.size());
+    arr("rank_nodes", rank_nodes, 10);
+    arr("rank_count", rank_count, 10);
+    arr("rank_ns", rank_ns, 10);
+    arr("rank_aborts", rank_aborts, 10);
+    printf("}\n");
+    fflush(stdout);
+  }
+};
+
 // WZ_JOIN22 (wz_match.cpp:2905-3104): the complete hash join over the Thm 2.2 space, with
 // the reference's output lines and exit codes (0 found, 3 exhausted).
 static int run_join22(int n, int a, int b, int c, int d, Clock::time_point T0) {
@@ -165,10 +216,19 @@ int main(int argc, char **argv) {
   if (const char *e = getenv("WZ_FH_AB_PROF")) cfg.ab_prof = atoi(e) != 0;
   cfg.orbit_qprune = env_ll("WZ_FH_ORBIT_QPRUNE", 0) != 0;
   if (env_ll("WZ_FH_CD_PRUNE", 0) || env_ll("WZ_FH_A2", 0) || env_ll("WZ_FH_WALL_SEC", 0) ||
-      env_set("WZ_FH_TELEMETRY") || env_set("WZ_FH_TARGET_C")) {
-    printf("RESULT: UNSUPPORTED FLAG (CD_PRUNE, A2, WALL_SEC, TELEMETRY, TARGET are not implemented)\n");
+      env_set("WZ_FH_TARGET_C")) {
+    printf("RESULT: UNSUPPORTED FLAG (CD_PRUNE, A2, WALL_SEC, TARGET are not implemented)\n");
     return 2;
   }
+  Telem TM;
+  if (const char *e = getenv("WZ_FH_TELEMETRY")) {   // wz_match.cpp:1545-1552
+    if (strcmp(e, "0") && strcmp(e, "1") && strcmp(e, "64")) {
+      fprintf(stderr, "WZ_FH_TELEMETRY must be 0, 1 or 64\n");
+      return 2;
+    }
+    TM.stride = atoi(e);
+  }
+  if (TM.stride) TM.depth_nodes.assign((n + 1) / 2 + 1, 0);
   const long long budget = env_ll("WZ_FH_AB_BUDGET", 200000);
   const long long max_cand = env_ll("WZ_FH_MAX_CAND", 0);
   const long long score_max = env_ll("WZ_FH_SCORE_MAX", 0);
@@ -432,6 +492,26 @@ int main(int argc, char **argv) {
     ck_pi = pi;
     if (!(resuming && pi == res_pi)) { ck_batch = 0; ck_k = 0; }
     const long long cand_at_start = cand;
+    // telemetry of this cell (wz_match.cpp:2532-2546, 2848-2853): a resumed cell replays the
+    // batches (or the ranks) before its checkpoint, and that replay is timed apart
+    const auto tm_cell_start = Clock::now();
+    bool tm_replay = TM.stride && resuming && pi == res_pi && (res_batch > 0 || res_k > 0);
+    if (tm_replay) TM.resume_cells++;
+    const long long tm_score0 = TM.score_ns_est, tm_sort0 = TM.sort_ns;
+    auto tm_end_replay = [&]() {
+      if (!tm_replay) return;
+      TM.resume_replay_ns += Telem::ns_since(tm_cell_start);
+      TM.resume_replay_score_ns_est += TM.score_ns_est - tm_score0;
+      TM.resume_replay_sort_ns += TM.sort_ns - tm_sort0;
+      tm_replay = false;
+    };
+    auto tm_cell_end = [&]() {
+      if (!TM.stride) return;
+      tm_end_replay();
+      TM.cell_ns += Telem::ns_since(tm_cell_start);
+      if (fh_stop) TM.cells_partial++;
+      else if (cand > cand_at_start) TM.cells_live_done++;
+    };
 
     if (cellsize > 0) {   // measurement: count only (wz_match.cpp:2749-2781, 2814-2836)
       for (long long j = b0; j < e0; j++) {
@@ -449,6 +529,7 @@ int main(int argc, char **argv) {
       printf("CDSTAT pi=%d dfs_nodes=%lld cd_prune=0 hall_fast=1\n", pi, cst.dfs_nodes);
       cells_done++;
       if (cand == cand_at_start) cells_empty++;
+      tm_cell_end();
       oi++;
       continue;
     }
@@ -462,6 +543,10 @@ int main(int argc, char **argv) {
     // checkpoint clock are looked at between slices (the reference looks after every
     // completion, :2686-2697): a slice is sized to take about slice_sec of device time
     // (doubling from 16 candidates while slices finish faster, halving when one is slower).
+    std::vector<long long> slice_depth;
+    long long slice_ns = 0;             // share of the current slice's time for the candidate at hand
+    const long long *slice_dn = nullptr;
+    int tm_rank = -1;                   // decile of the sorted rank (-1: unbuffered, no rank)
     auto complete_batch = [&](const std::vector<Buf> &items, size_t start, size_t stop_at, auto &&on_each) {
       std::vector<uint64_t> cd;
       std::vector<int> cellof;
@@ -474,13 +559,25 @@ int main(int argc, char **argv) {
         for (size_t ci = s0; ci < s1; ci++) { cd.push_back(items[ci].C); cd.push_back(items[ci].D); cellof.push_back(ab_prof ? pi : -1); }
         out.assign(s1 - s0, wzfh_completion{});
         const auto t0 = Clock::now();
-        int r = wzfh_complete(P, cd.data(), cellof.data(), (int)(s1 - s0), budget, out.data());
+        int r;
+        if (TM.stride) {   // whole walks with per-depth node counts (FH_TELEM depth_nodes)
+          slice_depth.assign((s1 - s0) * 64, 0);
+          r = wzfh_complete_depth(P, cd.data(), cellof.data(), (int)(s1 - s0), budget, out.data(), slice_depth.data());
+        } else {
+          r = wzfh_complete(P, cd.data(), cellof.data(), (int)(s1 - s0), budget, out.data());
+        }
         if (r != WZFH_OK && r != WZFH_EINTERNAL) { printf("RESULT: DEVICE ERROR (%s)\n", wzfh_last_error()); exit(2); }
-        const double dt = std::chrono::duration<double>(Clock::now() - t0).count();
+        const long long dns = Telem::ns_since(t0);
+        const double dt = 1e-9 * (double)dns;
         if (dt < 0.5 * slice_sec && s1 - s0 == slice_len) slice_len = std::min<size_t>(slice_len * 2, slice_max);
         else if (dt > 2 * slice_sec) slice_len = std::max<size_t>(slice_len / 2, 1);
-        for (size_t ci = s0; ci < s1; ci++)
+        const long long k = (long long)(s1 - s0);
+        for (size_t ci = s0; ci < s1; ci++) {
+          const long long i = (long long)(ci - s0);
+          slice_ns = dns * (i + 1) / k - dns * i / k;   // an even share of the slice's wall time
+          slice_dn = TM.stride ? &slice_depth[(size_t)i * 64] : nullptr;
           if (!on_each(ci, out[ci - s0])) return false;
+        }
         s0 = s1;
       }
       return true;
@@ -497,6 +594,25 @@ int main(int argc, char **argv) {
         return false;
       }
       ab_nodes += o.nodes;
+      if (TM.stride) {   // wz_match.cpp:2587-2614
+        const bool hist = TM.selected((uint64_t)TM.completions);
+        TM.complete_ns += slice_ns;
+        TM.completions++;
+        TM.total_nodes += o.nodes;
+        if (hist) {
+          TM.hist_candidates++;
+          TM.hist_nodes += o.nodes;
+          for (size_t d = 0; d < TM.depth_nodes.size(); d++) TM.depth_nodes[d] += slice_dn[d];
+        }
+        if (tm_rank >= 0) {
+          TM.rank_count[tm_rank]++;
+          TM.rank_nodes[tm_rank] += o.nodes;
+          TM.rank_ns[tm_rank] += slice_ns;
+          TM.rank_aborts[tm_rank] += (r == 3);
+        } else {
+          TM.unranked_count++; TM.unranked_nodes += o.nodes; TM.unranked_ns += slice_ns;
+        }
+      }
       if (cand_log) printf("CAND idx=%lld ci=%lld pi=%d r=%d nodes=%lld ns=0\n", cand, (long long)rank, pi, r, o.nodes);
       if (test_stop > 0 && ++test_ct >= test_stop) { g_sigterm = 1; fh_stop = true; }
       if (r == 1) pre_rej++;
@@ -506,15 +622,26 @@ int main(int argc, char **argv) {
         hit_idx = cand; hit_pi = pi; found = true; fh_stop = true;
         hit_nodes = o.nodes;
         solA = unpack(o.A, n + 1); solB = unpack(o.B, n + 1); solC = unpack(item.C, n); solD = unpack(item.D, n);
-        hit_score = wzfh_flat_score(n, item.C, item.D);
+        {   // the FIRSTHIT line's score is one more flat-score call (wz_match.cpp:2640)
+          const bool sample = TM.stride && TM.selected((uint64_t)TM.score_calls++);
+          const auto st0 = Clock::now();
+          hit_score = wzfh_flat_score(n, item.C, item.D);
+          if (sample) { TM.score_ns_est += Telem::ns_since(st0) * TM.stride; TM.score_samples++; }
+        }
         return false;
       }
       return !g_sigterm;
     };
     auto drain = [&]() {
-      if (resuming && pi == res_pi && cur_batch < res_batch) { cur_batch++; cellbuf.clear(); return; }
+      if (resuming && pi == res_pi && cur_batch < res_batch) {
+        cur_batch++;
+        cellbuf.clear();
+        if (cur_batch == res_batch && res_k == 0) tm_end_replay();   // the next buffer is new work
+        return;
+      }
       if (drain_top > 0 && cur_batch >= drain_batches) {
         resuming = false;
+        tm_end_replay();
         cellbuf.clear();
         if (!cell_stop) { cell_stop = true; cells_capped++; }
         return;
@@ -524,10 +651,14 @@ int main(int argc, char **argv) {
         start = (size_t)std::min<long long>((long long)cellbuf.size(), res_k);
         resuming = false;
       }
+      const auto sort_t0 = Clock::now();
       std::stable_sort(cellbuf.begin(), cellbuf.end(), [](const Buf &x, const Buf &y) { return x.sc < y.sc; });
+      if (TM.stride) TM.sort_ns += Telem::ns_since(sort_t0);
+      if (!resuming) tm_end_replay();
       size_t stop_at = cellbuf.size();
       if (drain_top > 0 && cur_batch < drain_batches) stop_at = std::min(cellbuf.size(), (size_t)drain_top);
       complete_batch(cellbuf, start, stop_at, [&](size_t ci, const wzfh_completion &o) {
+        tm_rank = TM.stride ? (int)(10 * ci / stop_at) : -1;
         bool go = complete_one(ci, cellbuf[ci], o);
         if (internal) return false;
         if (!found) { ck_batch = cur_batch; ck_k = (long long)ci + 1; maybe_ckpt(); }
@@ -565,7 +696,12 @@ int main(int argc, char **argv) {
         fprintf(dump, "\n");
       }
       long long sc = -1;
-      if (score_max > 0 || cell_order) sc = wzfh_flat_score(n, Cw, Dw);
+      if (score_max > 0 || cell_order) {
+        const bool sample = TM.stride && TM.selected((uint64_t)TM.score_calls++);
+        const auto st0 = Clock::now();
+        sc = wzfh_flat_score(n, Cw, Dw);
+        if (sample) { TM.score_ns_est += Telem::ns_since(st0) * TM.stride; TM.score_samples++; }
+      }
       if (score_max > 0 && sc > score_max) { score_rej++; continue; }
       if (cell_order) {
         cellbuf.push_back({sc, Cw, Dw, j - b0});
@@ -575,9 +711,14 @@ int main(int argc, char **argv) {
       }
       // non-buffered: completion order == eligible-stream order
       if (resuming && pi == res_pi) {
-        if (cell_done_ct < res_k) { cell_done_ct++; continue; }
+        if (cell_done_ct < res_k) {
+          cell_done_ct++;
+          if (cell_done_ct == res_k) tm_end_replay();
+          continue;
+        }
         resuming = false;
       }
+      tm_rank = -1;
       Buf item{sc, Cw, Dw, j - b0};
       std::vector<Buf> one{item};
       const long long rank = cell_done_ct;
@@ -597,6 +738,7 @@ int main(int argc, char **argv) {
       ck_pi = pi + nshard; ck_batch = 0; ck_k = 0;
       write_ckpt();
     }
+    tm_cell_end();
     oi++;
   }
   if (dump) fclose(dump);
@@ -635,6 +777,7 @@ int main(int argc, char **argv) {
   else printf("RESULT: NO HIT within probe budget — NOT a proof of absence\n");
   printf("Time: %gs\n", std::chrono::duration<double>(Clock::now() - T0).count());
   fflush(stdout);
+  TM.emit(n, TM.stride ? Telem::ns_since(T0) : 0);
   wzfh_destroy(P);
   return hit_idx >= 0 ? 0 : 3;
 }

@@ -43,3 +43,46 @@ def sample_cells(live, k, seed):
         j = i + next(rng) % (len(live) - i)
         live[i], live[j] = live[j], live[i]
     return sorted(live[:k])
+
+
+def _pack(v):
+    return sum(1 << i for i, x in enumerate(v) if x < 0)
+
+
+def pair_variants(X, Y):
+    """Images of a pair (X, Y) that keep N_X(s) + N_Y(s) at every shift s: negating either
+    sequence, reversing either one, and swapping them. Distinct packed (X, Y) words, in a fixed
+    order."""
+    out = []
+    for neg_x, neg_y, rev_x, rev_y, swap in ((0, 0, 0, 0, 0), (1, 0, 0, 0, 0), (0, 1, 0, 0, 0), (1, 1, 0, 0, 0),
+                                              (0, 0, 1, 0, 0), (0, 0, 0, 1, 0), (0, 0, 0, 0, 1), (1, 0, 1, 0, 1)):
+        x = [-v for v in X] if neg_x else list(X)
+        y = [-v for v in Y] if neg_y else list(Y)
+        x = x[::-1] if rev_x else x
+        y = y[::-1] if rev_y else y
+        if swap:
+            x, y = y, x
+        w = (_pack(x), _pack(y))
+        if w not in out:
+            out.append(w)
+    return out
+
+
+def join_workload(n, solution, nrand, seed):
+    """The join at table scale: nrand random C,D pairs (n elements) and nrand random A,B pairs
+    (n+1 elements) -- their autocorrelation vectors meet with vanishing probability -- with the
+    images of one known solution planted at seeded positions on both sides. Every planted A,B
+    image then hits the one table entry of the planted C,D images, and the solutions are exactly
+    the planted A,B images times the planted C,D images. Returns (cd, ab, planted_cd, planted_ab)
+    with cd, ab as uint64 arrays of shape (k, 2)."""
+    import numpy as np
+    A, B, C, D = solution
+    rng = np.random.default_rng(seed)
+    pcd, pab = pair_variants(C, D), pair_variants(A, B)
+    cd = rng.integers(0, 1 << n, size=(nrand + len(pcd), 2), dtype=np.uint64)
+    ab = rng.integers(0, 1 << (n + 1), size=(nrand + len(pab), 2), dtype=np.uint64)
+    for arr, planted in ((cd, pcd), (ab, pab)):
+        at = rng.choice(len(arr), size=len(planted), replace=False)
+        for i, (x, y) in zip(at, planted):
+            arr[i] = (x, y)
+    return cd, ab, pcd, pab

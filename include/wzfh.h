@@ -148,6 +148,15 @@ typedef struct {
 int wzfh_complete(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand,
                   long long budget, wzfh_completion *out);
 
+/* wzfh_complete with the nodes of each candidate split by depth: depth_nodes[64 i + d] = nodes
+ * counted at depth d (charges of the quad loop at depth d, the unwinding charges of an abort at
+ * their ancestor depth, and odd-L middle trials at depth (n+1)/2), so they sum to out[i].nodes.
+ * This is the depth histogram of the reference's FH_TELEM record (wz_match.cpp:1198, 1244).
+ * Every candidate is walked whole by one wavefront (no split), so it is slower than
+ * wzfh_complete; verdicts, nodes and A,B are the same. */
+int wzfh_complete_depth(wzfh_problem *p, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                        wzfh_completion *out, long long *depth_nodes);
+
 /* Allocation-free variant of wzfh_complete. No counterpart in the reference: it is for callers
  * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc). Same arguments, kernels and results as wzfh_complete, but
@@ -292,6 +301,17 @@ int wzfh_join_probe(wzfh_join *j, wzfh_join_result *out);
  * quadruples (A, B, C, D) of packed words to out in ascending order, or only counts when out
  * is NULL; WZFH_ECAPACITY (with *count set) when cap is smaller. Call after wzfh_join_probe. */
 int wzfh_join_solutions(wzfh_join *j, uint64_t *out, long long cap, long long *count);
+/* The same table from caller-given candidates instead of the profile streams: ncd C,D pairs
+ * (packed words, n elements; the earlier of two equal vectors keeps the entry, as stream order
+ * does) and nab A,B pairs (n+1 elements). These are the reference's insert and lookup loops
+ * (:2939-2950, :2985-2996) over an explicit list, for joins whose sides come from elsewhere
+ * (a cell search, a file of candidates) and for measuring the table at a given size. */
+int wzfh_join_build_from(wzfh_join *j, const uint64_t *cd, long long ncd, int slots_log2, wzfh_join_result *out);
+int wzfh_join_probe_from(wzfh_join *j, const uint64_t *ab, long long nab, wzfh_join_result *out);
+/* Device time (HIP events on the join's stream) of the table kernels of the last build (claim,
+ * write, verify rounds and the duplicate pass) and of the last probe kernel, in ms. No reference
+ * counterpart. */
+int wzfh_join_kernel_ms(const wzfh_join *j, double *build_ms, double *probe_ms);
 void wzfh_join_close(wzfh_join *j);
 
 /* ---- Exact checks ----------------------------------------------------------- */

@@ -28,6 +28,8 @@ Fixtures:
                       checkpoint bytes, an optional forgery of it, the resumed run's lines
                       and checkpoint bytes (the reference's own resume gates at n = 13, a
                       stale CFGSIG, n = 19 front-only interrupts)
+  telemetry.json      FH_TELEM v2 records (deterministic fields: counts, node sums, depth and rank
+                      histograms) of the reference's telemetry gate cases at strides 1 and 64
   qprune_cells.json   the Q-closure prune (WZ_FH_ORBIT_QPRUNE with the 64-element orbit
                       group): [qprune] counts, kept orbits, order-3 digests and the per-cell
                       streams of the cells left (n = 10..14 safe classes, n = 11, 13 ENDPOS)
@@ -484,6 +486,75 @@ def gen_qprune(jobs):
           f'{sum(r["qprune"][6] for r in recs)} cells removed by the prune')
 
 
+TELEM_KEYS = ('version', 'n', 'stride', 'resume_cells', 'score_calls', 'score_samples', 'completions',
+              'hist_candidates', 'hist_nodes', 'total_nodes', 'unranked_nodes', 'unranked_count', 'cells_live_done',
+              'cells_partial', 'depth_nodes', 'rank_nodes', 'rank_count', 'rank_aborts')
+
+
+def telem_record(out):
+    recs = [json.loads(l[len('FH_TELEM '):]) for l in out.splitlines() if l.startswith('FH_TELEM ')]
+    assert len(recs) == 1, out[-500:]
+    return {k: recs[0][k] for k in TELEM_KEYS}
+
+
+def telem_case(name, n, sig, cfg, stop_after=0, forge=None):
+    """A WZ_FH_TELEMETRY run (or an interrupt/resume pair whose resumed run has telemetry on):
+    the record's deterministic fields (counts, node sums, the depth and rank histograms; not the
+    timers) and the result lines."""
+    if stop_after:
+        base = {k: v for k, v in cfg.items() if k != 'WZ_FH_TELEMETRY'}
+        with tempfile.TemporaryDirectory() as td:
+            ck = Path(td)
+            run(n, sig, {**base, 'WZ_FH_CKPT_DIR': ck, 'WZ_FH_TEST_STOP_AFTER': stop_after})
+            p = ck / 'arm_0.ckpt'
+            if forge:
+                t = p.read_text()
+                for pat, rep in forge:
+                    t = re.sub(pat, rep, t)
+                p.write_text(t)
+            rc, out = run(n, sig, {**cfg, 'WZ_FH_CKPT_DIR': ck})
+    else:
+        rc, out = run(n, sig, cfg)
+    return dict(name=name, n=n, sig=list(sig), settings=cfg, stop_after=stop_after, forge=forge, rc=rc,
+                lines=semantic(out), telem=telem_record(out))
+
+
+def gen_telem(jobs):
+    """FH_TELEM v2 fixtures: the reference's own telemetry gate cases
+    (tools/test_firsthit_telemetry.py:94-171) at strides 1 and 64, with and without resume."""
+    common = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_FH_AB_PROF=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_ORBIT_CANON=1,
+                  WZ_FH_PROF_ORDER=1, WZ_FH_BUF_CAP=16, WZ_FH_DRAIN_TOP=5, WZ_FH_MAX_CAND=256,
+                  WZ_FH_PROG_SEC=9999, WZ_FH_OUTER_K=0)
+    cases = [((6, 5, 1, 0, 0), {}), ((11, 0, 6, 1, 3), {}), ((12, 1, 7, 0, 0), {}), ((13, 2, 4, 3, 5), {}),
+             ((13, 2, 4, 3, 5), {'WZ_FH_CELL_ORDER': 0}), ((11, 0, 6, 1, 3), {'WZ_FH_PROF_END': 2}),
+             ((13, 2, 4, 3, 5), {'WZ_FH_DRAIN_BATCHES': 2})]
+    tasks = []
+    for ci, (args, extra) in enumerate(cases):
+        for budget in (0, 1, 30):
+            for mode in (1, 64):
+                cfg = {**common, **extra, 'WZ_FH_AB_BUDGET': budget, 'WZ_FH_STREAM_REV': ci % 2,
+                       'WZ_FH_TELEMETRY': mode}
+                tasks.append((f'case{ci} budget={budget} stride={mode}', args[0], args[1:], cfg, 0, None))
+    for rev, buffered, stop in ((0, 1, 3), (1, 1, 3), (0, 0, 3), (1, 0, 7), (0, 1, 7), (1, 1, 10)):
+        cfg = {**common, 'WZ_FH_AB_BUDGET': 1, 'WZ_FH_STREAM_REV': rev, 'WZ_FH_CELL_ORDER': buffered,
+               'WZ_FH_TELEMETRY': 1}
+        tasks.append((f'resume rev={rev} buffered={buffered} stop={stop}', 13, (2, 4, 3, 5), cfg, stop, None))
+    for boundary in (0, 1):
+        cfg = {**common, 'WZ_FH_AB_BUDGET': 1, 'WZ_FH_BUF_CAP': 2, 'WZ_FH_DRAIN_TOP': 0, 'WZ_FH_TELEMETRY': 1}
+        tasks.append((f'boundary batch={boundary}', 13, (2, 4, 3, 5), cfg, 2,
+                      [(r'resume_batch=0\nresume_k=2\n', f'resume_batch={boundary}\nresume_k=0\n')]))
+    cfg = dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_ORBIT_CANON=1, WZ_FH_PROF_ORDER=1,
+               WZ_FH_AB_BUDGET=12, WZ_FH_BUF_CAP=64, WZ_FH_DRAIN_TOP=8, WZ_FH_MAX_CAND=3000,
+               WZ_FH_PROG_SEC=9999)
+    for sig in classes(19)[:2]:
+        for mode in (1, 64):
+            tasks.append((f'n19 stride={mode}', 19, sig, {**cfg, 'WZ_FH_TELEMETRY': mode}, 0, None))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: telem_case(*t), tasks))
+    (HERE / 'telemetry.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'telemetry.json: {len(recs)} runs, {sum(r["telem"]["completions"] for r in recs)} completions')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--only', default='')
@@ -493,7 +564,7 @@ def main():
         sys.exit(f'{REF} missing: run make -C oracle ref (needs the reference tree)')
     gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29,
                 solutions=gen_solutions, join=gen_join, join_signed=gen_join_signed, resume=gen_resume,
-                qprune=gen_qprune)
+                qprune=gen_qprune, telem=gen_telem)
     for name, g in gens.items():
         if not a.only or name in a.only.split(','):
             g(a.jobs)

@@ -69,3 +69,33 @@ def test_join_cli_lines_match_reference():
                 assert [int(x) for x in line.split(',')] == r[name]
         else:
             assert 'JOIN22 EXHAUSTED' in out
+
+
+def test_join_table_at_2_28_slots_finds_exactly_the_planted_solutions():
+    """The n = 44 join at the configuration's table size: 2^28 slots (44 int16 vector, tag, order and
+    C,D per slot, about 32 GB of HBM) filled to load 1/2 with random C,D pairs, probed with as many
+    random A,B pairs, with the images of the banked BS(45,44) planted on both sides. Distinct keys
+    must be the random pairs plus the one planted vector, the hits exactly the planted A,B images,
+    and the solution list exactly every planted A,B image with every planted C,D image, each with
+    NPAF = 0."""
+    from bs45_quantum_explorer_amd.workload import join_workload
+    sol = next(r for r in load('solutions.json') if r['n'] == 44 and not r['quarantined'])
+    nrand = (1 << 27) - 8
+    cd, ab, pcd, pab = join_workload(44, (sol['A'], sol['B'], sol['C'], sol['D']), nrand, seed=4428)
+    J = wz.Join(44, tuple(sol['sig']))
+    try:
+        t0 = time.time()
+        r = J.build_from(cd, slots_log2=28)
+        assert r.slots == 1 << 28 and r.cd_stream == len(cd)
+        assert r.distinct_keys == nrand + 1
+        r = J.probe_from(ab)
+        assert r.hits == len(pab) and r.found
+        sols = J.solutions()
+        want = sorted((a, b, c, d) for a, b in pab for c, d in pcd)
+        assert sols == want
+        for A, B, C, D in sols:
+            assert wz.npaf_max(44, unpack(A, 45), unpack(B, 45), unpack(C, 44), unpack(D, 44)) == 0
+        bms, pms = J.kernel_ms()
+        print(f'2^28-slot join: build {bms:.1f} ms, probe {pms:.1f} ms, {time.time() - t0:.1f} s wall', flush=True)
+    finally:
+        J.close()
