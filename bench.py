@@ -72,12 +72,10 @@ def key(r):
 
 
 def served_pass(server, reqs, timeout_s=600):
-    """Submit every (problem, cell) request, wait for all results; returns results by request
-    index and the wall time."""
+    """Submit every (problem, cell) request at once (one burst), wait for all results; returns
+    results by request index and the wall time."""
     t0 = time.perf_counter()
-    tick = {}
-    for i, (pi, c) in enumerate(reqs):
-        tick[server.submit(pi, c)] = i
+    tick = {t: i for i, t in enumerate(server.submit_batch(reqs))}
     got = [None] * len(reqs)
     n = 0
     deadline = time.monotonic() + timeout_s
@@ -171,9 +169,11 @@ def main():
     ap.add_argument('--join-slots-log2', type=int, default=28, help='hash-join table size (0: skip the join)')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch settings')
     ap.add_argument('--tune-cells', type=int, default=0,
-                    help='cells of the autotuning sample (0: one class slice, the batch a class launch sees)')
+                    help='cells of the autotuning sample (0: as many as a step, the batch a served stream launch '
+                         'sees: the server streams every class of one n in one launch)')
     ap.add_argument('--probe-traffic', action='store_true',
-                    help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + one batched search)')
+                    help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + two served steps)')
+    ap.add_argument('--tuning', default='', help='launch settings for --probe-traffic, as a bench line\'s autotune JSON')
     ap.add_argument('--traffic-summary', nargs=2, metavar=('FETCH_CSV', 'WRITE_CSV'),
                     help='summarize two rocprofv3 --pmc counter_collection.csv files into profiles/')
     ap.add_argument('--serve-sweep', action='store_true',
@@ -181,6 +181,7 @@ def main():
                          'saturation (its own JSON line)')
     ap.add_argument('--sweep-requests', type=int, default=600, help='requests per offered rate')
     ap.add_argument('--sweep-ab-budget', type=int, default=1 << 20)
+    ap.add_argument('--sweep-cell-blocks', default='', help='comma list of WZFH_CELL_BLOCKS values to sweep too')
     a = ap.parse_args()
     if a.traffic_summary:
         traffic_summary(*a.traffic_summary)
@@ -213,8 +214,7 @@ def main():
     tune = None
     if not a.no_tune:
         t0 = time.time()
-        sample = sample_cells(probs[0].live_cells(), a.tune_cells or len(slices[0]) // max(1, len(probs)),
-                              a.seed * 7919 + 1)
+        sample = sample_cells(probs[0].live_cells(), a.tune_cells or len(slices[0]), a.seed * 7919 + 1)
         t = probs[0].autotune(sample, a.top_k, a.ab_budget, **search)
         for P in probs[1:]:
             P.set_tuning(t)
@@ -365,19 +365,29 @@ def serve_sweep(a):
     setup_s = time.time() - t0
     search = dict(node_budget=a.node_budget, cand_cap=a.cand_cap)
     server = wz.Server(probs, a.top_k, a.sweep_ab_budget, max_batch=a.max_batch, **search)
-    rows, served = [], {}
+    rows, served, variants = [], {}, {}
     try:
-        got, dt = served_pass(server, pool[:a.sweep_requests])   # warm-up and closed-loop capacity
-        got, dt = served_pass(server, pool[:a.sweep_requests])
-        capacity = a.sweep_requests / dt
-        served.update({rq: r.result for rq, r in zip(pool, got)})
-        log(f'closed-loop capacity: {capacity:.1f} cells/s ({a.sweep_requests} requests in {dt:.2f} s)')
-        for frac in (0.2, 0.4, 0.6, 0.8, 1.0, 1.25):
-            rate = frac * capacity
-            lat, wall = poisson_latency(server, pool, rate, a.sweep_requests, a.seed + int(100 * frac))
-            rows.append({'offered_frac': frac, 'offered_per_s': round(rate, 2),
-                         'achieved_per_s': round(len(lat) / wall, 2), 'p50_ms': pct(lat, 50), 'p99_ms': pct(lat, 99)})
-            log(f'offered {rate:.1f}/s: {rows[-1]}')
+        for cb in [None] + [int(x) for x in a.sweep_cell_blocks.split(',') if x.strip()]:
+            if cb is not None:
+                os.environ['WZFH_CELL_BLOCKS'] = str(cb)
+            got, dt = served_pass(server, pool[:a.sweep_requests])   # warm-up and closed-loop capacity
+            got, dt = served_pass(server, pool[:a.sweep_requests])
+            cap_cb = a.sweep_requests / dt
+            served.update({rq: r.result for rq, r in zip(pool, got)})
+            log(f'cell blocks {cb}: closed-loop capacity {cap_cb:.1f} cells/s ({a.sweep_requests} requests in {dt:.2f} s)')
+            rows_cb = []
+            for frac in (0.2, 0.4, 0.6, 0.8, 1.0, 1.25):
+                rate = frac * cap_cb
+                lat, wall = poisson_latency(server, pool, rate, a.sweep_requests, a.seed + int(100 * frac))
+                rows_cb.append({'offered_frac': frac, 'offered_per_s': round(rate, 2),
+                                'achieved_per_s': round(len(lat) / wall, 2), 'p50_ms': pct(lat, 50),
+                                'p99_ms': pct(lat, 99)})
+                log(f'cell blocks {cb}: offered {rate:.1f}/s: {rows_cb[-1]}')
+            if cb is None:
+                capacity, rows = cap_cb, rows_cb
+            else:
+                variants[cb] = {'closed_loop_cells_per_s': round(cap_cb, 3), 'sweep': rows_cb}
+        os.environ.pop('WZFH_CELL_BLOCKS', None)
     finally:
         server.close()
     plain_lat, mism = [], 0
@@ -396,7 +406,8 @@ def serve_sweep(a):
             'closed_loop_cells_per_s': round(capacity, 3), 'plain_cells_per_s': round(plain_rate, 3),
             'served_over_plain': round(capacity / plain_rate, 2),
             'plain_latency_ms': {'p50': pct(plain_lat, 50), 'p99': pct(plain_lat, 99), 'cells': len(plain_lat)},
-            'sweep': rows, 'plain_vs_served_mismatch': mism, 'setup_s': round(setup_s, 1)}
+            'sweep': rows, 'cell_blocks_variants': variants, 'plain_vs_served_mismatch': mism,
+            'setup_s': round(setup_s, 1)}
     print(json.dumps(line), flush=True)
     for P in probs:
         P.close()
@@ -544,22 +555,44 @@ def _cpu_model():
 
 def probe_traffic(a):
     """For `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: first an elementwise kernel that
-    reads and writes exactly 256 MiB each (the counters' calibration), then one batched search
-    of the first class's cells of slice 0 (the batch a class launch sees in the served steps)
-    with the default launch settings."""
+    reads and writes exactly 256 MiB each (the counters' calibration), then two served steps of
+    the benchmark (slices 0 and 1 of the n = 44 set, every class) with the launch settings given
+    by --tuning (the autotune choice of a bench line), so every stream and completer dispatch in
+    the profile is one of the timed region's kind."""
     import torch
     x = torch.zeros(64 << 20, dtype=torch.int32, device='cuda')
     y = torch.empty_like(x)
     torch.add(x, 1, out=y)
     torch.cuda.synchronize()
-    P = wz.Problem(a.n, classes(a.n)[0], canon=1, order=1)
-    cells = sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000)[0::a.slices]
-    P.search(cells, a.top_k, a.ab_budget, cand_cap=a.cand_cap, node_budget=a.node_budget, batched=True)
-    tot = P.kernel_totals()
-    print(json.dumps({'calibration_bytes_each_way': 256 << 20, 'cells': len(cells),
+    cls = classes(a.n)[:a.classes]
+    probs, samples = [], []
+    for i, sig in enumerate(cls):
+        P = wz.Problem(a.n, sig, canon=1, order=1)
+        probs.append(P)
+        samples.append(sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000 + i))
+    if a.tuning:
+        want = json.loads(a.tuning)
+        for P in probs:
+            t = P.tuning()
+            for k, v in want.items():
+                if hasattr(t, k) and k not in ('trials', 'stream_ms', 'complete_ms'):
+                    setattr(t, k, v)
+            P.set_tuning(t)
+    server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=a.max_batch, cand_cap=a.cand_cap,
+                       node_budget=a.node_budget)
+    try:
+        for k in range(2):
+            reqs = [(ci, c) for ci, smp in enumerate(samples) for c in smp[k::a.slices]]
+            served_pass(server, reqs)
+        tot = server.kernel_totals()
+    finally:
+        server.close()
+    print(json.dumps({'calibration_bytes_each_way': 256 << 20, 'steps': 2,
                       'stream': {'ms': tot[0][0], 'units': tot[0][1], 'launches': tot[0][2]},
-                      'completer': {'ms': tot[1][0], 'units': tot[1][1], 'launches': tot[1][2]}}), flush=True)
-    P.close()
+                      'completer': {'ms': tot[1][0], 'units': tot[1][1], 'launches': tot[1][2]},
+                      'tuning': a.tuning}), flush=True)
+    for P in probs:
+        P.close()
 
 
 def traffic_summary(fetch_csv, write_csv):

@@ -24,7 +24,7 @@ EXPORTS = ('wzfh_config_default', 'wzfh_init', 'wzfh_init_host', 'wzfh_destroy',
            'wzfh_join_open', 'wzfh_join_build', 'wzfh_join_probe', 'wzfh_join_close',
            'wzfh_autotune', 'wzfh_get_tuning', 'wzfh_set_tuning', 'wzfh_kernel_totals', 'wzfh_complete_noalloc',
            'wzfh_serve_kernel_totals', 'wzfh_join_solutions', 'wzfh_qprune_stats', 'wzfh_complete_depth',
-           'wzfh_join_build_from', 'wzfh_join_probe_from', 'wzfh_join_kernel_ms')
+           'wzfh_join_build_from', 'wzfh_join_probe_from', 'wzfh_join_kernel_ms', 'wzfh_serve_submit_batch')
 NOALLOC_MAX_CAND = 1024            # WZFH_NOALLOC_MAX_CAND
 NOALLOC_SCRATCH_BYTES = 64 << 20   # WZFH_NOALLOC_SCRATCH_BYTES
 
@@ -136,6 +136,7 @@ def lib():
         L.wzfh_last_kernel_ms.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         L.wzfh_serve_open.argtypes = [vp, C.c_int, C.POINTER(ServeOpts), C.POINTER(vp)]
         L.wzfh_serve_submit.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong)]
+        L.wzfh_serve_submit_batch.argtypes = [vp, C.c_int, vp, vp, vp]
         L.wzfh_serve_poll.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int]
         L.wzfh_serve_close.argtypes = [vp]
         L.wzfh_autotune.argtypes = [vp, vp, C.c_int, C.POINTER(SearchOpts), C.POINTER(Tuning)]
@@ -332,6 +333,14 @@ class Server:
         t = C.c_longlong()
         _check(lib().wzfh_serve_submit(self.h, problem, cell, C.byref(t)))
         return t.value
+
+    def submit_batch(self, reqs):
+        """Submits [(problem, cell), ...] at once; returns their tickets in order."""
+        pr = np.ascontiguousarray(np.asarray([p for p, _ in reqs], dtype=np.int32))
+        ce = np.ascontiguousarray(np.asarray([c for _, c in reqs], dtype=np.int32))
+        tk = np.zeros(max(len(reqs), 1), np.int64)
+        _check(lib().wzfh_serve_submit_batch(self.h, len(reqs), pr.ctypes.data, ce.ctypes.data, tk.ctypes.data))
+        return [int(t) for t in tk[:len(reqs)]]
 
     def poll(self, max_results=1024, min_results=1, timeout_ms=1000):
         out = (ServeResult * max_results)()

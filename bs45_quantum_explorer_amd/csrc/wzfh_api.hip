@@ -484,7 +484,13 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     // best for one cell (71-80 ms against 90-130 ms at 2); batches take the tuned value
     int bpc = ncells <= 4 ? 4 : S->tune.blocks_per_cu;
     if (const char *e = getenv("WZFH_BLOCKS_PER_CU")) bpc = atoi(e);
-    const int blocks = std::max(1, S->num_cu * std::max(1, std::min(4, bpc)));
+    int blocks = std::max(1, S->num_cu * std::max(1, std::min(4, bpc)));
+    // WZFH_CELL_BLOCKS = k: at most k blocks per cell, so launches of a few cells leave the rest
+    // of the GPU to concurrent launches (serving many n at once); 0 = the whole grid
+    if (const char *e = getenv("WZFH_CELL_BLOCKS")) {
+      const long long k = atoll(e);
+      if (k > 0) blocks = (int)std::max(1ll, std::min<long long>(blocks, k * (long long)ncells));
+    }
     // idle polling backoff: waves per cell, so a few cells are not probed by thousands of
     // waves every iteration (the counters are device-scope atomics)
     a.scan_max = std::max(8, std::min(1024, (blocks * 4) / std::max(1, ncells)));

@@ -255,6 +255,24 @@ int wzfh_serve_submit(wzfh_server *S, int problem, int cell, long long *ticket) 
   return WZFH_OK;
 }
 
+int wzfh_serve_submit_batch(wzfh_server *S, int count, const int *problems, const int *cells, long long *tickets) {
+  if (!S || count < 0 || (count > 0 && (!problems || !cells))) return WZFH_EINVAL;
+  for (int i = 0; i < count; i++) {
+    if (problems[i] < 0 || problems[i] >= (int)S->problems.size()) return WZFH_EINVAL;
+    if (cells[i] < 0 || cells[i] >= wzfh_num_cells(S->problems[problems[i]])) return WZFH_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->closing) return WZFH_EINVAL;
+  const Clock::time_point now = Clock::now();
+  for (int i = 0; i < count; i++) {
+    ServeReq r{S->next_ticket++, problems[i], cells[i], now};
+    if (tickets) tickets[i] = r.ticket;
+    S->groups[S->group_of[problems[i]]].pending.push_back(r);
+  }
+  S->cv_work.notify_all();
+  return WZFH_OK;
+}
+
 int wzfh_serve_poll(wzfh_server *S, wzfh_serve_result *out, int max, int min_results, int timeout_ms) {
   if (!S || !out || max <= 0) return WZFH_EINVAL;
   std::unique_lock<std::mutex> lk(S->mu);

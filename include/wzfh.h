@@ -259,6 +259,9 @@ typedef struct {
 } wzfh_serve_result;
 int wzfh_serve_open(wzfh_problem **problems, int nproblems, const wzfh_serve_opts *opts, wzfh_server **out);
 int wzfh_serve_submit(wzfh_server *s, int problem, int cell, long long *ticket);
+/* Submits count requests at once (problems[i], cells[i]); tickets (may be NULL) receives their
+ * tickets in order. They arrive together, so one admission can take them all. */
+int wzfh_serve_submit_batch(wzfh_server *s, int count, const int *problems, const int *cells, long long *tickets);
 /* Waits up to timeout_ms for at least min_results finished requests; returns how many
  * were written to out (<= max), or a negative error. */
 int wzfh_serve_poll(wzfh_server *s, wzfh_serve_result *out, int max, int min_results, int timeout_ms);
