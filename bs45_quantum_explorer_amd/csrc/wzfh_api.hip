@@ -374,7 +374,7 @@ static int auto_split(const wzfh_problem &P, const wzfh_stream_opts &o, int ncel
 }
 
 int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o,
-                      wzfh_cell_stats *stats, StreamOut &out, const wzfh_problem *const *cell_prob) {
+                      wzfh_cell_stats *stats, StreamOut &out, const wzfh_problem *const *cell_prob, int cell_blocks) {
   DeviceState *S = P.dev;
   WZ_HIP(hipSetDevice(S->device));   // callers may be any host thread (serving workers)
   StreamArgs a = S->base;
@@ -485,12 +485,11 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     int bpc = ncells <= 4 ? 4 : S->tune.blocks_per_cu;
     if (const char *e = getenv("WZFH_BLOCKS_PER_CU")) bpc = atoi(e);
     int blocks = std::max(1, S->num_cu * std::max(1, std::min(4, bpc)));
-    // WZFH_CELL_BLOCKS = k: at most k blocks per cell, so launches of a few cells leave the rest
-    // of the GPU to concurrent launches (serving many n at once); 0 = the whole grid
-    if (const char *e = getenv("WZFH_CELL_BLOCKS")) {
-      const long long k = atoll(e);
-      if (k > 0) blocks = (int)std::max(1ll, std::min<long long>(blocks, k * (long long)ncells));
-    }
+    // at most cell_blocks blocks per cell (WZFH_CELL_BLOCKS overrides), so launches of a few
+    // cells leave the rest of the GPU to concurrent launches (serving many n at once)
+    long long kcell = cell_blocks;
+    if (const char *e = getenv("WZFH_CELL_BLOCKS")) kcell = atoll(e);
+    if (kcell > 0) blocks = (int)std::max(1ll, std::min<long long>(blocks, kcell * (long long)ncells));
     // idle polling backoff: waves per cell, so a few cells are not probed by thousands of
     // waves every iteration (the counters are device-scope atomics)
     a.scan_max = std::max(8, std::min(1024, (blocks * 4) / std::max(1, ncells)));
@@ -511,6 +510,8 @@ int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_
     a.don_any = (getenv("WZFH_DONATE_ANY") && atoi(getenv("WZFH_DONATE_ANY"))) ? 1 : 0;
     if (const char *e = getenv("WZFH_DONATE_LOW")) a.don_low = (uint32_t)std::max(0, atoi(e));
     a.don_max_level = std::min(21, std::max(0, a.half - 3));
+    a.don_keep = 0;
+    if (const char *e = getenv("WZFH_DONATE_KEEP")) a.don_keep = std::max(0, std::min(31, atoi(e)));
     WZ_HIP(hipEventRecord(S->ev[0], S->stream));
     WZ_HIP(launch_stream(a, blocks, S->stream));
     WZ_HIP(hipEventRecord(S->ev[1], S->stream));

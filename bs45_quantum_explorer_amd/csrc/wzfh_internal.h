@@ -78,8 +78,10 @@ struct StreamOut {
 };
 // cell_prob (optional): per cell slot, the problem whose cell list cells[i] indexes (problems
 // of one n with the same stream tables, see stream_key); null: every cell is P's.
+// cell_blocks > 0: at most that many persistent blocks per cell (a launch of a few cells then
+// leaves most of the GPU to concurrent launches of other problems); 0: the whole grid.
 int run_stream(wzfh_problem &P, const int *cells, int ncells, const wzfh_stream_opts &o, wzfh_cell_stats *stats,
-               StreamOut &out, const wzfh_problem *const *cell_prob = nullptr);
+               StreamOut &out, const wzfh_problem *const *cell_prob = nullptr, int cell_blocks = 0);
 // The flattest K candidates of every cell of a stream batch, in sorted order per cell.
 struct TopK {
   std::vector<uint64_t> cd;          // packed (C, D) pairs

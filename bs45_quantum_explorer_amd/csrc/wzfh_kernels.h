@@ -66,6 +66,8 @@ struct StreamArgs {
   uint32_t don_hops;              // capped: generations of frontier donations that donate again
   int don_any;                    // 1: every item may donate (also under a cap)
   int don_max_level;              // deepest level whose siblings are donated
+  int don_keep;                   // capped: > 0 donates every level from the shallowest live one down
+                                  // to depth d-1-don_keep (deep items first); 0 = the shallowest only
   int *cell_lock, *cell_crossed;  // walker try-lock; 1 once the cut is known
   // capped launches hand out a cell's static items through its own cursor, at most
   // static_win items ahead of the walk, so lanes work near each cell's frontier

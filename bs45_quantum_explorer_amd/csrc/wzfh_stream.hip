@@ -708,18 +708,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
       since = 0;
       {
         // remaining live siblings per level, from the residuals found by undoing the path
+        // (8 bits per level, levels 0..31 in four words)
         uint64_t rx = RX, ry = RY;
         int best = -1;
-        unsigned bmask = 0;
+        uint64_t lm0 = 0, lm1 = 0, lm2 = 0, lm3 = 0;
         for (int lev = d - 1; lev >= base; lev--) {
           const int b = (int)((dig >> (3 * (lev - base))) & 7);
           quad_res(s_di[lev], s_q8[b], -1, rx, ry);
           const int hi_e = ((dmask >> lev) & 1u) ? b : ((lev == base) ? hi_b : 7);
           const unsigned m = child_mask(s_ne, s_eq, s_di[lev], rx, ry) & (0xFEu << b) & (0xFFu >> (7 - hi_e));
-          if (m) { best = lev; bmask = m; }
+          if (m) best = lev;
+          const uint64_t put = (uint64_t)m << (8 * (lev & 7));
+          if (lev < 8) lm0 |= put; else if (lev < 16) lm1 |= put; else if (lev < 24) lm2 |= put; else lm3 |= put;
         }
+        auto lmask = [&](int lev) -> unsigned {
+          const uint64_t w = lev < 8 ? lm0 : lev < 16 ? lm1 : lev < 24 ? lm2 : lm3;
+          return (unsigned)((w >> (8 * (lev & 7))) & 255u);
+        };
+        // Under a cap the subtrees at the shallowest live level lie mostly beyond the cell's cut;
+        // with don_keep > 0 every level from there down to the deepest live level above
+        // d-1-don_keep is given away, deepest first, so the first new items are the work that
+        // comes next in DFS order. The donor keeps its subtree and the levels below `top`.
+        int top = best;
+        if (best >= 0 && capped && a.don_keep > 0)
+          for (int lev = min(d - 1 - a.don_keep, a.don_max_level); lev > best; lev--)
+            if (lmask(lev)) { top = lev; break; }
         if (best >= 0 && best <= a.don_max_level) {
-          const int k = __popc(bmask);
+          int k = 0;
+          for (int lev = top; lev >= best; lev--) k += __popc(lmask(lev));
           const unsigned long long al = __hip_atomic_load(&a.dq[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (al + (unsigned long long)k <= (unsigned long long)a.rec_cap &&
               atomicCAS(&a.dq[0], al, al + (unsigned long long)k) == al) {
@@ -733,38 +749,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void s
             } else {
               key = a.it_key[rec];
             }
-            uint32_t t_new = tail;   // dead entries after the lane's digits above best
+            uint32_t t_new = tail;   // dead entries after the last new item: levels above best
             for (int lev = base; lev < best; lev++) {
               const int b = (int)((dig >> (3 * (lev - base))) & 7);
-              key |= key_put(lev, b);
               const int hi_e = ((dmask >> lev) & 1u) ? b : ((lev == base) ? hi_b : 7);
               t_new += (uint32_t)(hi_e - b);
             }
-            const int bb = (int)((dig >> (3 * (best - base))) & 7);
-            const int hi_best = ((dmask >> best) & 1u) ? bb : ((best == base) ? hi_b : 7);
             const uint32_t after = a.it_next[rec];
             const uint32_t child_hops = front ? (uint32_t)a.don_hops : (hops > 0 ? hops - 1 : 0u);
-            unsigned m = bmask;
-            int lo = bb + 1;
-            for (int i = 0; i < k; i++) {
-              m &= m - 1;   // drop this item's live sibling; the next one bounds its range
-              const int nxt = m ? (__ffs(m) - 1) : -1;
-              const int hi = m ? nxt - 1 : hi_best;
-              const uint32_t r = (uint32_t)al + (uint32_t)i;
-              a.it_cell[r] = (uint32_t)cs;
-              a.it_key[r] = key;
-              a.it_meta[r] = (uint32_t)best | ((uint32_t)lo << 8) | ((uint32_t)hi << 12) | (child_hops << 16);
-              a.it_tail[r] = m ? 0u : t_new;
-              a.it_next[r] = m ? r + 1 : after;
-              a.it_cands[r] = 0;
-              a.it_noff[r] = kUnsetOff;
-              lo = nxt;
+            // items in DFS order: level top's live siblings, ..., level best's. Each covers one
+            // live sibling and the dead ones up to the next live one at its level (the last one
+            // of a level to the level's end); the dead entries of a level with no live sibling
+            // between top and best become the tail of the item before them.
+            uint32_t r = (uint32_t)al, last = (uint32_t)al, pend = 0;
+            bool any = false;
+            for (int lev = top; lev >= best; lev--) {
+              const int bl = (int)((dig >> (3 * (lev - base))) & 7);
+              const int hi_l = ((dmask >> lev) & 1u) ? bl : ((lev == base) ? hi_b : 7);
+              unsigned m = lmask(lev);
+              if (!m) { pend += (uint32_t)(hi_l - bl); continue; }
+              if (any) a.it_tail[last] = pend;
+              pend = 0;
+              uint64_t key_l = key;   // the item's prefix, then the lane's digits down to lev
+              for (int j = base; j < lev; j++) key_l |= key_put(j, (int)((dig >> (3 * (j - base))) & 7));
+              int lo = bl + 1;
+              while (m) {
+                m &= m - 1;   // drop this item's live sibling; the next one bounds its range
+                const int nxt = m ? (__ffs(m) - 1) : -1;
+                const int hi = m ? nxt - 1 : hi_l;
+                a.it_cell[r] = (uint32_t)cs;
+                a.it_key[r] = key_l;
+                a.it_meta[r] = (uint32_t)lev | ((uint32_t)lo << 8) | ((uint32_t)hi << 12) | (child_hops << 16);
+                a.it_tail[r] = 0u;
+                a.it_next[r] = r + 1;
+                a.it_cands[r] = 0;
+                a.it_noff[r] = kUnsetOff;
+                last = r;
+                any = true;
+                r++;
+                lo = nxt;
+              }
             }
-            a.it_next[rec] = (uint32_t)al;   // the donor's run now ends with its current subtree
+            a.it_tail[last] = t_new;
+            a.it_next[last] = after;
+            a.it_next[rec] = (uint32_t)al;   // the donor's run now ends below level top
             for (int i = 0; i < k; i++)
               __hip_atomic_store(&a.it_state[(uint32_t)al + (uint32_t)i], rec_tag(ep, 1), __ATOMIC_RELEASE,
                                  __HIP_MEMORY_SCOPE_AGENT);
-            dmask |= ((2u << best) - 1u) & ~((1u << base) - 1u);
+            dmask |= ((2u << top) - 1u) & ~((1u << base) - 1u);
             tail = 0;
             atomicAdd(&a.dq[4], 1ull);   // progress
             if (telem) { atomicAdd(&s_tel[6], 1ull); atomicAdd(&s_tel[7], (unsigned long long)k); }

@@ -125,6 +125,7 @@ inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new double(0); return 0; 
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return 0; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { *e = emu_now(); return 0; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(*b - *a); return 0; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { return 0; }
 inline hipError_t hipMalloc(void **p, size_t b) {
   *p = calloc(1, b);
   if (*p && getenv("EMU_STALE_FILL")) {   // reused memory: fill with a stale 32-bit value

@@ -73,4 +73,23 @@ for rec in [r for r in stream_runs() if r['n'] == 8][:1]:
             got = [row[0], st.cand, st.leaves, st.hall_ok, st.dfs_nodes, h]
             report(got == list(row), 'stream n8', rec['sig'], 'cell', row[0])
     S.close()
+# the same, under a node budget with multi-level frontier donation (every level from the shallowest
+# live one down to d-2 given away at once, deepest first), against the oracle's budgeted stream
+from oracle_lib import OracleProblem  # noqa: E402
+os.environ['WZFH_DONATE_KEEP'] = '1'
+n12 = next(r for r in stream_runs() if r['n'] == 12 and r['canon'] == 1 and r['rev'] == 0)
+S = wz.Problem(12, n12['sig'], m6=1, thm211b=1, thm212=1, canon=1, order=1)
+O = OracleProblem(12, n12['sig'], canon=1, order=1)
+cells = S.live_cells()[:2]
+for budget in (37, 3000):
+    st, offs, cd = S.stream(cells, node_budget=budget, split_depth=1, window=2)
+    for i, pi in enumerate(cells):
+        got, cands, ost = O.stream_budget(pi, 0, budget, 100000)
+        lines = [''.join(f'{int(v)} ' for v in c[0]) + '| ' + ''.join(f'{int(v)} ' for v in c[1]) + '\n' for c in cands]
+        ok = (st[i].cand, st[i].dfs_nodes) == (got, ost.dfs_nodes) and wz.dump_lines(12, cd[offs[i]:offs[i + 1]]) == lines
+        report(ok, 'stream n12 multi-level donation, budget', budget, 'cell', pi, (st[i].cand, st[i].dfs_nodes),
+               (got, ost.dfs_nodes))
+S.close()
+O.close()
+os.environ.pop('WZFH_DONATE_KEEP', None)
 print('bad', bad, flush=True)

@@ -205,14 +205,18 @@ def test_stream_cap_and_budget_together_match_oracle():
         P.close()
 
 
-@pytest.mark.parametrize('cap,budget', [(0, 0), (0, 37), (0, 4096), (3, 0), (3, 200)])
-def test_stream_forced_donation_matches_oracle(cap, budget, monkeypatch):
+@pytest.mark.parametrize('cap,budget,keep', [(0, 0, 0), (0, 37, 0), (0, 4096, 0), (3, 0, 0), (3, 200, 0),
+                                             (0, 37, 1), (0, 4096, 1), (0, 4096, 3), (3, 200, 1), (3, 0, 2)])
+def test_stream_forced_donation_matches_oracle(cap, budget, keep, monkeypatch):
     """Running items donate their untried subtrees every 2 DFS moves: item lists, offsets,
-    tails and the cut must still reproduce the oracle's stream exactly."""
+    tails and the cut must still reproduce the oracle's stream exactly. keep > 0: under a cap a
+    donation gives away every level from the shallowest live one down to depth d-1-keep at once
+    (deepest first), so items with tails between levels are exercised."""
     from oracle_lib import OracleProblem
     monkeypatch.setenv('WZFH_DONATE_MOVES', '2')
     monkeypatch.setenv('WZFH_DONATE_LOW', '1000000')
     monkeypatch.setenv('WZFH_DONATE_ANY', '1')
+    monkeypatch.setenv('WZFH_DONATE_KEEP', str(keep))
     n, sig = 19, (2, 4, 3, 7)
     P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1)
     O = OracleProblem(n, sig, canon=1, order=1)
