@@ -39,6 +39,12 @@ from pathlib import Path
 
 import numpy as np
 
+# One hardware queue per HIP stream for the server's streams (one per problem plus one per n):
+# with HIP's default of 4 queues, the completer launches of the 12 classes share 4 queues and
+# wait for each other (measured on the n = 44 bench: 653 served cells/s with 4 queues, 716 with 8,
+# 845 with 16). Read by the HIP runtime when it starts, so it is set before the library loads.
+os.environ.setdefault('GPU_MAX_HW_QUEUES', '16')
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -280,6 +286,8 @@ def main():
     # ---- the leaf test against FP64 on a verify sample (slice 1): every certain decision
     verify = verify_sample(a, probs, slices[1 % a.slices])
     log(f'verify: {verify}')
+    util = stream_utilization(probs[0], sample_cells(probs[0].live_cells(), len(slices[0]), a.seed * 7919 + 2), a)
+    log(f'stream utilization: {util}')
 
     streamed = sum(r.stream.cand for r in res_timed)
     completed = sum(r.completed for r in res_timed)
@@ -290,9 +298,9 @@ def main():
     traffic = load_traffic()
     sums = [list(tot[k]) for k in range(2)]
     roof = {
-        'stream': kernel_roofline('wzfh::stream_kernel', OPS_PER_CD_NODE,
-                                  f'{OPS_PER_CD_NODE} INT32 ops per counted C,D DFS node (rec() entry)',
-                                  *sums[0], timed_wall, traffic.get('wzfh::stream_kernel')),
+        'stream': dict(kernel_roofline('wzfh::stream_kernel', OPS_PER_CD_NODE,
+                                       f'{OPS_PER_CD_NODE} INT32 ops per counted C,D DFS node (rec() entry)',
+                                       *sums[0], timed_wall, traffic.get('wzfh::stream_kernel')), utilization=util),
         'completer': kernel_roofline('wzfh::complete_kernel', ops_per_ab_node(a.n),
                                      f'{ops_per_ab_node(a.n)} INT32 ops (6L+n) per counted completer node',
                                      *sums[1], timed_wall, traffic.get('wzfh::complete_kernel')),
@@ -481,6 +489,30 @@ def verify_sample(a, probs, reqs):
                 os.environ[k] = v
     return {'cells': cells, 'leaves_tested': leaves, 'decided_in_fp64': uncertain, 'false_decisions': bad,
             'margin': round(probs[0].last_kernel(10)[0], 4)}
+
+
+def stream_utilization(P, cells, a):
+    """One stream launch of a step-sized batch of one class with the kernel's counters on
+    (WZFH_TELEMETRY=1): DFS moves its lanes made, main-loop passes of its waves, and the share of
+    the moves that are counted nodes inside the cells' budgets (the rest is speculation past the
+    cuts, abandoned)."""
+    keep = os.environ.get('WZFH_TELEMETRY')
+    os.environ['WZFH_TELEMETRY'] = '1'
+    try:
+        st, _, _ = P.stream(cells, node_budget=a.node_budget, want_cands=False)
+        moves, passes = P.last_kernel(11)[1], P.last_kernel(12)[1]
+        ms = P.last_kernel(0)[0]
+    finally:
+        if keep is None:
+            os.environ.pop('WZFH_TELEMETRY', None)
+        else:
+            os.environ['WZFH_TELEMETRY'] = keep
+    inner = P.tuning().inner_moves
+    counted = sum(x.dfs_nodes for x in st)
+    return {'cells': len(cells), 'launch_ms': round(ms, 2), 'lane_moves': moves, 'wave_passes': passes,
+            'moves_per_wave_pass': round(moves / passes, 2) if passes else None,
+            'lane_slot_use': round(moves / (passes * 64 * inner), 5) if passes else None,
+            'counted_share_of_moves': round(counted / moves, 4) if moves else None}
 
 
 def cpu_cores():

@@ -1561,15 +1561,21 @@ int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_sear
       if (ms < best_ms) { best_ms = ms; best = t; }
     }
   }
-  // completer split depth, on the sample's own top-K candidates
+  // completer split depth, then the charge cap of split items (the longest item a launch waits
+  // for: a lower cap means shorter launches and more rounds), on the sample's own top-K candidates
   wzfh_search_opts so = *opts;
   so.batched = 1;
   std::vector<wzfh_search_result> ref_res;
   double best_c = -1;
-  for (int ext : {best.complete_split, 3, 4, 5}) {
-    if (best_c >= 0 && ext == best.complete_split) continue;
+  std::vector<std::pair<int, int>> ctrials;   // (knob 0 split / 1 cap, value)
+  for (int ext : {best.complete_split, 3, 4, 5}) ctrials.push_back({0, ext});
+  for (int lg : {10, 12, 16, 0}) ctrials.push_back({1, lg});
+  for (size_t ti = 0; ti < ctrials.size(); ti++) {
+    const int knob = ctrials[ti].first, val = ctrials[ti].second;
+    if (ti > 0 && ((knob == 0 && val == best.complete_split) || (knob == 1 && val == best.complete_cap_log2))) continue;
     wzfh_tuning t = best;
-    t.complete_split = ext;
+    if (knob == 0) t.complete_split = val;
+    else t.complete_cap_log2 = val;
     S->tune = t;
     std::vector<wzfh_search_result> res(ncells);
     double ms = 0;
@@ -1584,10 +1590,10 @@ int wzfh_autotune(wzfh_problem *p, const int *cells, int ncells, const wzfh_sear
       if (res[i].hit != ref_res[i].hit || res[i].ab_nodes != ref_res[i].ab_nodes || res[i].A != ref_res[i].A ||
           res[i].B != ref_res[i].B || res[i].clean_no != ref_res[i].clean_no || res[i].aborted != ref_res[i].aborted) {
         S->tune = best;
-        set_error("autotune: completer results differ between split depths");
+        set_error("autotune: completer results differ between launch settings");
         return WZFH_EKERNEL;
       }
-    if (best_c < 0 || ms < best_c) { best_c = ms; best.complete_split = ext; }
+    if (best_c < 0 || ms < best_c) { best_c = ms; best = t; }
   }
   best.stream_ms = best_ms;
   best.complete_ms = best_c;

@@ -211,7 +211,7 @@ int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_
 /* ---- Autotuning ----------------------------------------------------------------
  * No reference counterpart. Launch settings that change only speed, never results: the
  * stream kernel's DFS moves per lane per pass, the capped launches' donation generations
- * and static-item window, and the completer's split depth. wzfh_autotune times them on a
+ * and static-item window, and the completer's split depth and item charge cap. wzfh_autotune times them on a
  * sample of cells with the caller's search options (coordinate descent, one knob at a
  * time), checks that every trial returns the same per-cell results as the first (else
  * WZFH_EINTERNAL), keeps the fastest in the problem handle and returns it. Environment
