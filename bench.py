@@ -15,9 +15,16 @@ every computation is the real one, results are exact.
 
 A step = one slice of that set: slice k holds cells j = k (mod 12) of every class's seeded
 sample (~1,024 cells across the 12 classes), so 12 consecutive steps search the whole
-12 x 1,024 set once; step s (warmup steps first) takes slice s mod 12. The step is served:
-every cell is submitted to the continuous-batching server and the step ends when every
-result is back. `value` is served cells/s over the timed steps. On the same JSON line:
+12 x 1,024 set once; step s (warmup steps first) takes slice s mod 12. The steps are served:
+each step's cells are submitted to the continuous-batching server as one burst, with four steps
+in flight (--in-flight): step s+4 is submitted when every result of step s is back, so the
+stream launches of later steps run while earlier steps' completions finish, as under continuous
+load; a stream launch admits at most one step. The timed region starts with nothing in flight
+and ends when the last result of the last timed step is back. `value` is served cells/s over
+the timed steps (cells / that wall time). Launch settings (speed only; every trial is checked to
+return the same results) come from the library's autotuner on a step-sized sample of one class,
+then the completer's item charge cap from a few served steps per candidate cap. On the same
+JSON line:
   plain     one cell per search call (the single-cell GPU path), cells/s and latency
   batched   one persistent launch per class on slice 0 (the kernel-level view)
   latency   open-loop Poisson arrivals at --load x the served rate: p50 / p99 cell latency
@@ -96,6 +103,48 @@ def served_pass(server, reqs, timeout_s=600):
     return got, time.perf_counter() - t0
 
 
+def served_steps(server, steps, depth, timeout_s=600):
+    """Serve a sequence of steps (lists of requests) back to back with at most `depth` steps in
+    flight: steps 0..depth-1 are submitted at once, and step s+depth as soon as every result of
+    step s is back, so the server always holds the next step while one completes (a server
+    under continuous load: one step's stream launch runs while the previous step's completions
+    finish). Returns per step its results by request index and its submit-to-last-result time,
+    and the wall time of the whole sequence."""
+    t0 = time.perf_counter()
+    got = [[None] * len(st) for st in steps]
+    left = [len(st) for st in steps]
+    sub_t, dts = [0.0] * len(steps), [0.0] * len(steps)
+    where = {}
+
+    def submit(s):
+        sub_t[s] = time.perf_counter()
+        for i, t in enumerate(server.submit_batch(steps[s])):
+            where[t] = (s, i)
+
+    nxt = 0
+    while nxt < min(depth, len(steps)):
+        submit(nxt)
+        nxt += 1
+    done = 0
+    deadline = time.monotonic() + timeout_s
+    while done < len(steps):
+        if time.monotonic() > deadline:
+            raise RuntimeError(f'served steps: {done} of {len(steps)} steps after {timeout_s} s')
+        for r in server.poll(max_results=4096, min_results=1, timeout_ms=2000):
+            if r.status != 0:
+                raise RuntimeError(f'served request failed with status {r.status}')
+            s, i = where.pop(r.ticket)
+            got[s][i] = r
+            left[s] -= 1
+            if left[s] == 0:
+                dts[s] = time.perf_counter() - sub_t[s]
+                done += 1
+                if nxt < len(steps):
+                    submit(nxt)
+                    nxt += 1
+    return got, dts, time.perf_counter() - t0
+
+
 def poisson_latency(server, reqs, rate, count, seed):
     """Open-loop Poisson arrivals at `rate` requests/s; latency = finish - submit."""
     rng = random.Random(seed)
@@ -166,10 +215,15 @@ def main():
     ap.add_argument('--plain-cells', type=int, default=2, help='cells per class timed one per call')
     ap.add_argument('--latency-requests', type=int, default=1500)
     ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')
-    ap.add_argument('--max-batch', type=int, default=4096)
+    ap.add_argument('--max-batch', type=int, default=0, help='cells per stream launch (0: one slice)')
+    ap.add_argument('--in-flight', type=int, default=4, help='served steps in flight at once')
+    ap.add_argument('--serve-caps', default='11,12,13,14',
+                    help='completer charge caps (log2) tried on served steps after autotuning (empty: skip)')
+    ap.add_argument('--serve-tune-steps', type=int, default=4, help='served steps per cap trial')
     ap.add_argument('--seed', type=int, default=44)
     ap.add_argument('--verify-cells', type=int, default=32, help='cells per class in the FP64 verify sample')
-    ap.add_argument('--cpu-stride', type=int, default=2, help='CPU sample: every k-th cell of slice 0 per class')
+    ap.add_argument('--cpu-slices', type=int, default=2, help='CPU sample: the cells of slices 0 .. k-1')
+    ap.add_argument('--cpu-stride', type=int, default=1, help='CPU sample: every k-th of those cells per class')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--join-slots-log2', type=int, default=28, help='hash-join table size (0: skip the join)')
@@ -179,7 +233,7 @@ def main():
                          'sees: the server streams every class of one n in one launch)')
     ap.add_argument('--probe-traffic', action='store_true',
                     help='run the HBM-traffic probe for rocprofv3 --pmc (calibration kernel + two served steps)')
-    ap.add_argument('--tuning', default='', help='launch settings for --probe-traffic, as a bench line\'s autotune JSON')
+    ap.add_argument('--tuning', default='', help='launch settings (a bench line\'s autotune JSON) instead of autotuning')
     ap.add_argument('--traffic-summary', nargs=2, metavar=('FETCH_CSV', 'WRITE_CSV'),
                     help='summarize two rocprofv3 --pmc counter_collection.csv files into profiles/')
     ap.add_argument('--serve-sweep', action='store_true',
@@ -218,7 +272,11 @@ def main():
     # a class launch sees) and applied to every class; the tuner checks that each setting
     # returns identical results
     tune = None
-    if not a.no_tune:
+    if a.tuning:
+        apply_tuning(probs, a.tuning)
+        tune = dict(json.loads(a.tuning), source='--tuning')
+        log(f'launch settings given: {tune}')
+    elif not a.no_tune:
         t0 = time.time()
         sample = sample_cells(probs[0].live_cells(), a.tune_cells or len(slices[0]), a.seed * 7919 + 1)
         t = probs[0].autotune(sample, a.top_k, a.ab_budget, **search)
@@ -228,30 +286,37 @@ def main():
         log(f'autotune: {tune}')
     log(f'setup: {len(cls)} classes, {sum(map(len, samples))} cells in {a.slices} slices, {setup_s:.1f} s')
 
+    # one stream launch per step: a launch admits at most one slice, so the next step queued
+    # behind it waits for the next launch
+    max_batch = a.max_batch or max(len(sl) for sl in slices)
+    if a.serve_caps and not a.tuning:
+        tune = served_cap_tune(a, probs, slices, search, max_batch, tune)
+
     # ---- served (the measured value)
     served = {}
-    server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=a.max_batch, **search)
+    server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=max_batch, **search)
     try:
-        for w in range(a.warmup):
-            reqs = slices[w % a.slices]
-            got, dt = served_pass(server, reqs)
-            served.update({rq: r.result for rq, r in zip(reqs, got)})
-            log(f'served warmup {w + 1}/{a.warmup} (slice {w % a.slices}): {len(reqs)} cells, {dt * 1e3:.1f} ms')
+        if a.warmup:
+            wsteps = [slices[w % a.slices] for w in range(a.warmup)]
+            got, dts, _ = served_steps(server, wsteps, a.in_flight)
+            for w, (reqs, g) in enumerate(zip(wsteps, got)):
+                served.update({rq: r.result for rq, r in zip(reqs, g)})
+                log(f'served warmup {w + 1}/{a.warmup} (slice {w % a.slices}): {len(reqs)} cells, {dts[w] * 1e3:.1f} ms')
         server.kernel_totals(reset=True)
-        times, ncells, res_timed = [], 0, []
+        tsteps = [slices[s % a.slices] for s in range(a.steps)]
         t_timed = time.perf_counter()
-        for s in range(a.steps):
-            reqs = slices[s % a.slices]
-            got, dt = served_pass(server, reqs)
-            times.append(dt)
-            ncells += len(reqs)
-            res_timed += [r.result for r in got]
-            served.update({rq: r.result for rq, r in zip(reqs, got)})
-            log(f'served step {s + 1}/{a.steps} (slice {s % a.slices}): {len(reqs)} cells, {dt * 1e3:.1f} ms')
-        timed_wall = time.perf_counter() - t_timed
+        got, dts, timed_wall = served_steps(server, tsteps, a.in_flight)
         tot = server.kernel_totals()
-        step_s = float(np.mean(times))
-        served_rate = ncells / float(np.sum(times))
+        ncells, res_timed = 0, []
+        for s, (reqs, g) in enumerate(zip(tsteps, got)):
+            ncells += len(reqs)
+            res_timed += [r.result for r in g]
+            served.update({rq: r.result for rq, r in zip(reqs, g)})
+            log(f'served step {s + 1}/{a.steps} (slice {s % a.slices}): {len(reqs)} cells, {dts[s] * 1e3:.1f} ms '
+                f'from submit to last result')
+        log(f'served: {a.steps} steps, {ncells} cells in {timed_wall:.3f} s ({a.in_flight} in flight)')
+        step_s = timed_wall / max(1, a.steps)
+        served_rate = ncells / timed_wall
         # ---- open-loop latency at a fraction of the served rate, over the whole set
         everything = [rq for sl in slices for rq in sl]
         rate = max(1.0, a.load * served_rate)
@@ -310,7 +375,7 @@ def main():
     join, jroof = (None, {}) if not a.join_slots_log2 else join_scale(a)
     roof.update(jroof)
 
-    cpu = None if a.no_cpu else cpu_baseline(a, cls, probs, slices[0], served)
+    cpu = None if a.no_cpu else cpu_baseline(a, cls, probs, [rq for k in range(a.cpu_slices) for rq in slices[k]], served)
 
     line = {
         'metric': 'cells/sec & C,D cands/sec at n=44 (served vs plain GPU); p50/p99 cell latency',
@@ -330,10 +395,11 @@ def main():
         'plain_cells_per_s': round(plain_rate, 3),
         'served_over_plain': round(served_rate / plain_rate, 2) if plain_rate else None,
         'batched_cells_per_s': round(nbatched / batched_s, 3) if batched_s else None,
-        'cands_streamed_per_s': round(streamed / float(np.sum(times)), 1),
-        'cands_completed_per_s': round(completed / float(np.sum(times)), 1),
-        'dfs_nodes_per_s': round(dfs_nodes / float(np.sum(times)), 1),
-        'ab_nodes_per_s': round(ab_nodes / float(np.sum(times)), 1),
+        'cands_streamed_per_s': round(streamed / timed_wall, 1),
+        'cands_completed_per_s': round(completed / timed_wall, 1),
+        'dfs_nodes_per_s': round(dfs_nodes / timed_wall, 1),
+        'ab_nodes_per_s': round(ab_nodes / timed_wall, 1),
+        'step_latency_ms': {'p50': pct(dts, 50), 'max': round(1e3 * max(dts), 1), 'in_flight': a.in_flight},
         'served_latency_ms': {'p50': pct(lat, 50), 'p99': pct(lat, 99), 'offered_per_s': round(rate, 2),
                               'load': a.load, 'requests': len(lat)},
         'plain_latency_ms': {'p50': pct(plain_lat, 50), 'p99': pct(plain_lat, 99), 'cells': nplain},
@@ -372,7 +438,7 @@ def serve_sweep(a):
         pool.append((pi, live[pi][rng.randrange(len(live[pi]))]))
     setup_s = time.time() - t0
     search = dict(node_budget=a.node_budget, cand_cap=a.cand_cap)
-    server = wz.Server(probs, a.top_k, a.sweep_ab_budget, max_batch=a.max_batch, **search)
+    server = wz.Server(probs, a.top_k, a.sweep_ab_budget, max_batch=a.max_batch or 4096, **search)
     rows, served, variants = [], {}, {}
     try:
         for cb in [None] + [int(x) for x in a.sweep_cell_blocks.split(',') if x.strip()]:
@@ -526,8 +592,8 @@ def cpu_cores():
 
 def cpu_baseline(a, cls, probs, reqs, served):
     """The oracle port (oracle/wz_oracle.cpp, wzo_cell_search) on the host cores: the same cell
-    unit on a bounded sample of the same cells (every --cpu-stride-th cell of slice 0 per class),
-    one cell per thread, results compared with the served ones cell by cell."""
+    unit on a bounded sample of the same cells (every --cpu-stride-th cell of the first --cpu-slices
+    slices, per class), one cell per thread, results compared with the served ones cell by cell."""
     sys.path.insert(0, str(ROOT / 'tests'))
     try:
         from oracle_lib import OracleProblem
@@ -550,26 +616,30 @@ def cpu_baseline(a, cls, probs, reqs, served):
         return ci, c, r
 
     t0 = time.perf_counter()
-    cands = nodes = mism = 0
+    cands = nodes = mism = compared = 0
     with ThreadPoolExecutor(threads) as ex:
         for k, (ci, c, r) in enumerate(ex.map(one, work)):
             cands += r.cand
             nodes += r.dfs_nodes
-            g = served[(ci, c)]
+            if (k + 1) % 64 == 0:
+                log(f'cpu baseline: {k + 1}/{len(work)} cells')
+            g = served.get((ci, c))
+            if g is None:   # a cell the served steps did not reach: timed, not compared
+                continue
+            compared += 1
             want = (r.cand, r.dfs_nodes, r.completed, r.hit, r.hit_rank, r.ab_nodes, r.clean_no, r.aborted)
             got = (g.stream.cand, g.stream.dfs_nodes, g.completed, g.hit, g.hit_rank, g.ab_nodes, g.clean_no,
                    g.aborted)
             if want != got or (r.hit and (list(wz.unpack(g.A, a.n + 1)) != list(r.A[:a.n + 1]) or
                                           list(wz.unpack(g.B, a.n + 1)) != list(r.B[:a.n + 1]))):
                 mism += 1
-            if (k + 1) % 64 == 0:
-                log(f'cpu baseline: {k + 1}/{len(work)} cells')
     dt = time.perf_counter() - t0
     model = _cpu_model()
     return {'value': round(len(work) / dt, 4), 'unit': 'cells/s', 'cores': threads, 'kind': 'port',
-            'sample': f'{len(work)} cells (every {a.cpu_stride}th cell of slice 0 of each class) of the same '
+            'sample': f'{len(work)} cells ({"every" if a.cpu_stride == 1 else f"every {a.cpu_stride}th"} cell of slices '
+                      f'0..{a.cpu_slices - 1} of each class) of the same '
                       f'workload and unit, one cell per thread',
-            'mismatches_vs_served': mism, 'cells_compared': len(work),
+            'mismatches_vs_served': mism, 'cells_compared': compared,
             'cands_per_s': round(cands / dt, 1), 'dfs_nodes_per_s': round(nodes / dt, 1), 'seconds': round(dt, 2),
             'setup_s': round(setup, 1), 'cpu_model': model, 'host': host,
             'port_vs_reference': 'profiles/cpu_port_vs_reference.md'}
@@ -583,6 +653,51 @@ def _cpu_model():
     except OSError:
         pass
     return None
+
+
+def served_cap_tune(a, probs, slices, search, max_batch, tune):
+    """The completer's item charge cap for the served pipeline. The library's autotuner times one
+    class's batched search, where a launch waits for its longest item; served steps of all classes
+    overlap those tails, so a lower cap (shorter items, more launches) can serve faster. Each cap of
+    --serve-caps (and the autotuner's) serves --serve-tune-steps steps (slices 0, 1, ...) with
+    --in-flight steps in flight on a fresh server; the fastest is kept on every problem. Every cap
+    must return the same per-cell results as the first, else this raises."""
+    t0 = time.time()
+    base = probs[0].tuning().as_dict()
+    caps = [base['complete_cap_log2']] + [int(x) for x in a.serve_caps.split(',') if x.strip()]
+    caps = list(dict.fromkeys(caps))
+    steps = [slices[k % a.slices] for k in range(a.serve_tune_steps)]
+    ref, rates = {}, {}
+    for cap in caps:
+        apply_tuning(probs, json.dumps(dict(base, complete_cap_log2=cap)))
+        server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=max_batch, **search)
+        try:
+            got, _, wall = served_steps(server, steps, a.in_flight)
+        finally:
+            server.close()
+        for reqs, g in zip(steps, got):
+            for rq, r in zip(reqs, g):
+                if ref.setdefault(rq, key(r.result)) != key(r.result):
+                    raise RuntimeError(f'served cap tuning: cap 2^{cap} changes the result of cell {rq}')
+        rates[cap] = round(sum(map(len, steps)) / wall, 1)
+        log(f'served cap 2^{cap}: {rates[cap]} cells/s')
+    best = max(rates, key=rates.get)
+    apply_tuning(probs, json.dumps(dict(base, complete_cap_log2=best)))
+    out = dict(tune or {}, complete_cap_log2=best)
+    out['served_cap_trials'] = {str(k): v for k, v in rates.items()}
+    out['served_cap_seconds'] = round(time.time() - t0, 1)
+    return out
+
+
+def apply_tuning(probs, text):
+    """Sets the launch settings named in `text` (a bench line's autotune JSON) on every problem."""
+    want = json.loads(text)
+    for P in probs:
+        t = P.tuning()
+        for k, v in want.items():
+            if hasattr(t, k) and k not in ('trials', 'stream_ms', 'complete_ms'):
+                setattr(t, k, v)
+        P.set_tuning(t)
 
 
 def probe_traffic(a):
@@ -603,19 +718,12 @@ def probe_traffic(a):
         probs.append(P)
         samples.append(sample_cells(P.live_cells(), a.cells_per_class, a.seed * 1000 + i))
     if a.tuning:
-        want = json.loads(a.tuning)
-        for P in probs:
-            t = P.tuning()
-            for k, v in want.items():
-                if hasattr(t, k) and k not in ('trials', 'stream_ms', 'complete_ms'):
-                    setattr(t, k, v)
-            P.set_tuning(t)
-    server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=a.max_batch, cand_cap=a.cand_cap,
-                       node_budget=a.node_budget)
+        apply_tuning(probs, a.tuning)
+    steps = [[(ci, c) for ci, smp in enumerate(samples) for c in smp[k::a.slices]] for k in range(2)]
+    server = wz.Server(probs, a.top_k, a.ab_budget, max_batch=a.max_batch or max(map(len, steps)),
+                       cand_cap=a.cand_cap, node_budget=a.node_budget)
     try:
-        for k in range(2):
-            reqs = [(ci, c) for ci, smp in enumerate(samples) for c in smp[k::a.slices]]
-            served_pass(server, reqs)
+        served_steps(server, steps, a.in_flight)   # as the timed region serves them
         tot = server.kernel_totals()
     finally:
         server.close()
