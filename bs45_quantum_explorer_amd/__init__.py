@@ -85,7 +85,7 @@ class KernelTotals(C.Structure):
 
 
 class ServeOpts(C.Structure):
-    _fields_ = [('search', SearchOpts), ('max_batch', C.c_int), ('reserved', C.c_int)]
+    _fields_ = [('search', SearchOpts), ('max_batch', C.c_int), ('complete_workers', C.c_int)]
 
 
 class ServeResult(C.Structure):
@@ -320,11 +320,11 @@ class Server:
     """Continuous-batching cell-search server over a list of Problems (wzfh_serve_*)."""
 
     def __init__(self, problems, top_k, budget, cand_cap=0, node_budget=0, max_batch=256, split_depth=0,
-                 window=0):
+                 window=0, complete_workers=0):
         self.problems = list(problems)
         arr = (C.c_void_p * len(self.problems))(*[p.h for p in self.problems])
         opts = ServeOpts(SearchOpts(StreamOpts(cand_cap, node_budget, split_depth, window), top_k, 1, budget),
-                         max_batch, 0)
+                         max_batch, complete_workers)
         h = C.c_void_p()
         _check(lib().wzfh_serve_open(arr, len(self.problems), C.byref(opts), C.byref(h)))
         self.h = h

@@ -116,6 +116,8 @@ struct DeviceState {
   void *noalloc = nullptr;         // this device's copy of `scratch` (allocation-free completer)
   DBuf telem;                      // stream-kernel counters (WZFH_TELEMETRY=1)
   unsigned long long last_telem[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long long walked = 0;            // completer nodes walked by all items of a run (WZFH_TELEMETRY=1)
+  long long last_walked = 0;
   double hall_eps = 0.0;           // margin of the matrix-core leaf test
   // the last stream result, reused when the caller asks again for the same cells and
   // options (count first, then fetch) so the kernel runs once
@@ -339,8 +341,7 @@ int device_init(wzfh_problem &P) {
   return WZFH_OK;
 }
 
-void device_free(wzfh_problem &P) {
-  DeviceState *S = P.dev;
+static void free_state(DeviceState *S) {
   if (!S) return;
   for (DBuf *b : {&S->telem, &S->hall_frag, &S->hcos, &S->hsin, &S->order, &S->rej, &S->outer2, &S->outer3, &S->rows, &S->kab,
                   &S->cell_res, &S->cell_rootok, &S->cands, &S->ctr, &S->cell_tot, &S->kept,
@@ -355,7 +356,61 @@ void device_free(wzfh_problem &P) {
   for (auto &e : S->ev) if (e) (void)hipEventDestroy(e);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
+}
+
+void device_free(wzfh_problem &P) {
+  free_state(P.dev);
   P.dev = nullptr;
+}
+
+// A device state for completer runs only, with its own HIP stream and buffers and copies of P's
+// completer tables (profile rows, the A,B bound table, the A1 outer tables): a serving loop runs
+// several completion runs of one problem side by side on such states.
+static int complete_state_init(const wzfh_problem &P, DeviceState *S) {
+  const DeviceState *D = P.dev;
+  S->device = D->device;
+  S->num_cu = D->num_cu;
+  S->complete_lds_max = D->complete_lds_max;
+  S->tune = D->tune;
+  S->kab_host = D->kab_host;
+  WZ_HIP(hipSetDevice(S->device));
+  WZ_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  for (auto &e : S->ev) WZ_HIP(hipEventCreate(&e));
+  if (S->outer2.ensure(sizeof kOuter.t2) || S->outer3.ensure(sizeof kOuter.t3)) return WZFH_ENOMEM;
+  WZ_HIP(hipMemcpy(S->outer2.p, kOuter.t2, sizeof kOuter.t2, hipMemcpyHostToDevice));
+  WZ_HIP(hipMemcpy(S->outer3.p, kOuter.t3, sizeof kOuter.t3, hipMemcpyHostToDevice));
+  if (!P.rows.empty()) {
+    if (S->rows.ensure(P.rows.size() * sizeof(AbRow))) return WZFH_ENOMEM;
+    WZ_HIP(hipMemcpy(S->rows.p, P.rows.data(), P.rows.size() * sizeof(AbRow), hipMemcpyHostToDevice));
+  }
+  if (S->kab.ensure(S->kab_host.size())) return WZFH_ENOMEM;
+  WZ_HIP(hipMemcpy(S->kab.p, S->kab_host.data(), S->kab_host.size(), hipMemcpyHostToDevice));
+  return WZFH_OK;
+}
+
+int complete_state(const wzfh_problem &P, DeviceState **out) {
+  *out = nullptr;
+  if (!P.dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
+  DeviceState *S = new DeviceState();
+  const int rc = complete_state_init(P, S);
+  if (rc != WZFH_OK) { free_state(S); return rc; }
+  *out = S;
+  return WZFH_OK;
+}
+
+void complete_state_free(DeviceState *S) { free_state(S); }
+
+void complete_state_sync_tuning(const wzfh_problem &P, DeviceState *S) {
+  if (P.dev && S) S->tune = P.dev->tune;
+}
+
+int state_kernel_totals(DeviceState *S, wzfh_kernel_sums *out, int reset) {
+  std::lock_guard<std::mutex> lk(S->tot_mu);
+  for (int k = 0; k < 2; k++) {
+    if (out) { out->ms[k] = S->tot_ms[k]; out->units[k] = S->tot_units[k]; out->launches[k] = S->tot_launches[k]; }
+    if (reset) { S->tot_ms[k] = 0; S->tot_units[k] = 0; S->tot_launches[k] = 0; }
+  }
+  return WZFH_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -704,13 +759,12 @@ static int complete_buffers(DeviceState *S, Carve *carve, int nent, size_t nw, s
   return WZFH_OK;
 }
 
-static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff, const int *rlen,
+static int complete_launch(wzfh_problem &P, DeviceState *S, const uint64_t *cd, const int *roff, const int *rlen,
                            const std::vector<CompleteEntry> &ents, int maxrows, long long budget, int ext,
                            long long item_cap, int *v,
                            long long *nd, uint64_t *A, uint64_t *B, int *fj, long long *fcc, long long *fcur,
                            double *ms, Carve *carve, std::vector<std::vector<ItemRes>> *items,
                            std::vector<long long> *depth = nullptr) {
-  DeviceState *S = P.dev;
   const int nent = (int)ents.size();
   const int words = std::max(1, (maxrows + 63) / 64);
   // first-round launches under the root canon enumerate depth-0 digits 6 and 7 only
@@ -812,6 +866,12 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
     WZ_HIP(hipStreamSynchronize(S->stream));
     return WZFH_OK;
   };
+  if (getenv("WZFH_TELEMETRY") && atoi(getenv("WZFH_TELEMETRY"))) {   // every item's walk, kept or not
+    std::vector<long long> wn(nw);
+    WZ_HIP(hipMemcpyAsync(wn.data(), a.nodes, nw * 8, hipMemcpyDeviceToHost, S->stream));
+    WZ_HIP(hipStreamSynchronize(S->stream));
+    for (long long x : wn) S->walked += x;
+  }
   if (a.depth_out) {
     depth->assign(nw * 64, 0);
     WZ_HIP(hipMemcpyAsync(depth->data(), a.depth_out, nw * 64 * 8, hipMemcpyDeviceToHost, S->stream));
@@ -903,8 +963,8 @@ static int complete_launch(wzfh_problem &P, const uint64_t *cd, const int *roff,
 // items (8^3 per candidate) so one candidate's tree runs on many waves; the subtree that holds
 // a budget abort is split the same way again (3 more levels) until it is small, then walked as
 // one item that counts the abort exactly.
-static int complete_split_ext(const wzfh_problem &P) {
-  int ext = (P.L1 / 2 >= 6) ? P.dev->tune.complete_split : 0;
+static int complete_split_ext(const wzfh_problem &P, const DeviceState *S) {
+  int ext = (P.L1 / 2 >= 6) ? S->tune.complete_split : 0;
   if (const char *e = getenv("WZFH_COMPLETE_SPLIT")) ext = std::max(0, std::min(6, atoi(e)));
   if (ext >= P.L1 / 2) ext = 0;
   return ext;
@@ -912,9 +972,9 @@ static int complete_split_ext(const wzfh_problem &P) {
 // Split items stop after 2^complete_cap_log2 charges (the tuning; WZFH_COMPLETE_ITEM_CAP = C
 // overrides with C charges, 0 = no cap); the subtree of a capped item is split again in the next
 // round, so no item walks much more than the cap.
-static long long item_charge_cap(const wzfh_problem &P) {
+static long long item_charge_cap(const DeviceState *S) {
   if (const char *e = getenv("WZFH_COMPLETE_ITEM_CAP")) return std::max(0ll, atoll(e));
-  const int lg = P.dev->tune.complete_cap_log2;
+  const int lg = S->tune.complete_cap_log2;
   return lg > 0 ? 1ll << std::min(lg, 40) : 0;
 }
 
@@ -931,7 +991,7 @@ long long serve_cell_slots(wzfh_problem &P, int nshare, const wzfh_search_opts &
   size_t free_b = 0, total_b = 0;
   if (!P.dev || hipSetDevice(P.dev->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
     return 1ll << 30;
-  const int ext = complete_split_ext(P);
+  const int ext = complete_split_ext(P, P.dev);
   const double items = ext > 0 ? (double)(1ll << (3 * ext)) : 1.0;
   const double cands = o.stream.cand_cap > 0 ? (double)o.stream.cand_cap + 512.0 : 1024.0;
   const double per_cell = 16384.0 * 72.0 + cands * 64.0 + (double)std::max(1, o.top_k) * items * 44.0 + 65536.0;
@@ -976,8 +1036,8 @@ struct Frame {
 }  // namespace
 
 int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
-                 wzfh_completion *out, Carve *carve, long long *depth_nodes) {
-  DeviceState *S = P.dev;
+                 wzfh_completion *out, Carve *carve, long long *depth_nodes, DeviceState *dev) {
+  DeviceState *S = dev ? dev : P.dev;
   if (ncand <= 0) return WZFH_OK;
   WZ_HIP(hipSetDevice(S->device));
   std::vector<int> roff(ncand, -1), rlen(ncand, 0);
@@ -995,12 +1055,13 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
   }
   const bool limited = budget > 0;
   std::vector<Result> fin(ncand, Result{2, 0, 0, 0, 0, 0, false});
+  S->walked = 0;
   double ms = 0;
   // per-depth counts need whole walks: one item per candidate
-  const int ext0 = depth_nodes ? 0 : complete_split_ext(P);
+  const int ext0 = depth_nodes ? 0 : complete_split_ext(P, S);
   const int max_base = std::min(P.L1 / 2 - 1, 21);   // paths are 3 bits per depth in 64 bits
   const long long exact_item = exact_item_charges();
-  const long long cap = ext0 > 0 ? item_charge_cap(P) : 0;
+  const long long cap = ext0 > 0 ? item_charge_cap(S) : 0;
   std::vector<Frame> frames;
   std::vector<Job> jobs, next_jobs;
   for (int i = 0; i < ncand; i++) jobs.push_back(Job{i, 0, 0, budget, 0, 0, -1, i, ext0 == 0});
@@ -1083,7 +1144,7 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
         std::vector<uint64_t> A(ne), B(ne);
         std::vector<std::vector<ItemRes>> items;
         std::vector<long long> dep;
-        int rc = complete_launch(P, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, icap, v.data(),
+        int rc = complete_launch(P, S, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, icap, v.data(),
                                  nd.data(), A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms, carve, &items,
                                  depth_nodes ? &dep : nullptr);
         if (rc != WZFH_OK) return rc;
@@ -1137,6 +1198,7 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
     jobs.swap(next_jobs);
   }
   S->last_ms[1] = ms;
+  S->last_walked = S->walked;
   long long units = 0;
   bool internal = false;
   for (int i = 0; i < ncand; i++) {
@@ -1407,7 +1469,12 @@ int wzfh_npaf_check(int n, const int8_t *A, const int8_t *B, const int8_t *C, co
 }
 
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units) {
-  if (!p || !p->dev || kind < 0 || kind > 12) return WZFH_EINVAL;
+  if (!p || !p->dev || kind < 0 || kind > 13) return WZFH_EINVAL;
+  if (kind == 13) {   // completer nodes walked by the last run's items (WZFH_TELEMETRY=1)
+    if (ms) *ms = (double)p->dev->last_walked;
+    if (units) *units = p->dev->last_walked;
+    return WZFH_OK;
+  }
   if (kind == 10) {   // margin of the matrix-core leaf test
     if (ms) *ms = p->dev->hall_eps;
     if (units) *units = 0;
@@ -1427,13 +1494,7 @@ int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *
 int wzfh_kernel_totals(wzfh_problem *p, wzfh_kernel_sums *out, int reset) {
   if (!p) return WZFH_EINVAL;
   if (!p->dev) { set_error("problem was set up without a device"); return WZFH_ENODEV; }
-  DeviceState *S = p->dev;
-  std::lock_guard<std::mutex> lk(S->tot_mu);
-  for (int k = 0; k < 2; k++) {
-    if (out) { out->ms[k] = S->tot_ms[k]; out->units[k] = S->tot_units[k]; out->launches[k] = S->tot_launches[k]; }
-    if (reset) { S->tot_ms[k] = 0; S->tot_units[k] = 0; S->tot_launches[k] = 0; }
-  }
-  return WZFH_OK;
+  return state_kernel_totals(p->dev, out, reset);
 }
 
 int wzfh_search_cells(wzfh_problem *p, const int *cells, int ncells, const wzfh_search_opts *opts,

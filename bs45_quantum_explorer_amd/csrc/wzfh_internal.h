@@ -92,8 +92,16 @@ void select_topk(int n, const StreamOut &so, int ncells, int K, TopK &t);
 // Per-cell drain counters from the completions of t (results must hold the stream stats).
 void aggregate_completions(const TopK &t, const wzfh_completion *comp, wzfh_search_result *res);
 struct Carve;
+// dev (optional): run on this device state (complete_state) instead of P's own.
 int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
-                 wzfh_completion *out, Carve *carve = nullptr, long long *depth_nodes = nullptr);
+                 wzfh_completion *out, Carve *carve = nullptr, long long *depth_nodes = nullptr,
+                 DeviceState *dev = nullptr);
+// A device state for completer runs of P only (own HIP stream and buffers, copies of P's
+// completer tables), so several completion runs of one problem can run side by side.
+int complete_state(const wzfh_problem &P, DeviceState **out);
+void complete_state_free(DeviceState *S);
+void complete_state_sync_tuning(const wzfh_problem &P, DeviceState *S);   // copies P's launch settings
+int state_kernel_totals(DeviceState *S, wzfh_kernel_sums *out, int reset);
 // Problems whose stream kernels use identical tables (same n, residue modulus, stream order,
 // ENDPOS, side and pinned roots) can share one stream launch.
 std::string stream_key(const wzfh_problem &P);

@@ -61,6 +61,9 @@ struct Group {
 
 struct wzfh_server {
   std::vector<wzfh_problem *> problems;
+  int workers = 1;                           // completion runs per problem at once
+  std::vector<std::vector<wzfh::DeviceState *>> comp_states;   // per problem: states of workers 1..
+
   wzfh_serve_opts opts;
   std::mutex mu;
   std::condition_variable cv_work, cv_comp, cv_done;
@@ -158,8 +161,12 @@ static void stream_loop(wzfh_server *S, int g) {
   }
 }
 
-static void complete_loop(wzfh_server *S, int pidx) {
+// Completion worker w of problem pidx: worker 0 runs on the problem's own device state, the
+// others on states of their own, so a batch that arrives while a run waits for its longest
+// items starts at once on another worker.
+static void complete_loop(wzfh_server *S, int pidx, int w) {
   wzfh_problem *P = S->problems[pidx];
+  wzfh::DeviceState *dev = w > 0 ? S->comp_states[pidx][w - 1] : nullptr;
   const int g = S->group_of[pidx];
   while (true) {
     std::vector<CompBatch> batches;
@@ -181,7 +188,9 @@ static void complete_loop(wzfh_server *S, int pidx) {
     }
     const int nk = (int)cell_of.size();
     std::vector<wzfh_completion> comp(nk);
-    int rc = nk ? wzfh::run_complete(*P, cd.data(), cell_of.data(), nk, S->opts.search.budget, comp.data())
+    if (dev) wzfh::complete_state_sync_tuning(*P, dev);
+    int rc = nk ? wzfh::run_complete(*P, cd.data(), cell_of.data(), nk, S->opts.search.budget, comp.data(), nullptr,
+                                     nullptr, dev)
                 : WZFH_OK;
     const std::string msg = rc != WZFH_OK ? wzfh_last_error() : "";
     const auto finished = Clock::now();
@@ -234,16 +243,31 @@ int wzfh_serve_open(wzfh_problem **problems, int nproblems, const wzfh_serve_opt
     S->group_of[i] = it->second;
     S->groups[it->second].members.push_back(i);
   }
+  // one worker per problem by default: on served n = 44 steps (12 problems) two or three workers
+  // were 10-20 % slower than one (more HIP streams than hardware queues, and more completer waves
+  // beside the stream launches)
+  S->workers = opts->complete_workers > 0 ? std::min(opts->complete_workers, 8) : 1;
+  S->comp_states.resize(nproblems);
+  auto undo = [&](int rc) {
+    for (Group &H : S->groups) if (H.sh) wzfh_destroy(H.sh);
+    for (auto &v : S->comp_states) for (wzfh::DeviceState *d : v) wzfh::complete_state_free(d);
+    delete S;
+    return rc;
+  };
   for (Group &G : S->groups) {
     const int rc = wzfh::stream_handle(*problems[G.members[0]], &G.sh);
-    if (rc != WZFH_OK) {
-      for (Group &H : S->groups) if (H.sh) wzfh_destroy(H.sh);
-      delete S;
-      return rc;
-    }
+    if (rc != WZFH_OK) return undo(rc);
   }
+  for (int i = 0; i < nproblems; i++)
+    for (int w = 1; w < S->workers; w++) {
+      wzfh::DeviceState *d = nullptr;
+      const int rc = wzfh::complete_state(*problems[i], &d);
+      if (rc != WZFH_OK) return undo(rc);
+      S->comp_states[i].push_back(d);
+    }
   for (int g = 0; g < (int)S->groups.size(); g++) S->stream_threads.emplace_back(stream_loop, S, g);
-  for (int i = 0; i < nproblems; i++) S->comp_threads.emplace_back(complete_loop, S, i);
+  for (int i = 0; i < nproblems; i++)
+    for (int w = 0; w < S->workers; w++) S->comp_threads.emplace_back(complete_loop, S, i, w);
   *out = S;
   return WZFH_OK;
 }
@@ -297,15 +321,24 @@ int wzfh_serve_kernel_totals(wzfh_server *S, wzfh_kernel_sums *out, int reset) {
   std::vector<wzfh_problem *> hs;
   for (Group &G : S->groups) hs.push_back(G.sh);
   hs.insert(hs.end(), S->problems.begin(), S->problems.end());
-  for (wzfh_problem *h : hs) {
-    wzfh_kernel_sums t{};
-    if (int rc = wzfh_kernel_totals(h, &t, reset)) return rc;
+  auto add = [&](const wzfh_kernel_sums &t) {
     for (int k = 0; k < 2; k++) {
       sum.ms[k] += t.ms[k];
       sum.units[k] += t.units[k];
       sum.launches[k] += t.launches[k];
     }
+  };
+  for (wzfh_problem *h : hs) {
+    wzfh_kernel_sums t{};
+    if (int rc = wzfh_kernel_totals(h, &t, reset)) return rc;
+    add(t);
   }
+  for (auto &v : S->comp_states)   // the extra completion workers' states
+    for (wzfh::DeviceState *d : v) {
+      wzfh_kernel_sums t{};
+      wzfh::state_kernel_totals(d, &t, reset);
+      add(t);
+    }
   if (out) *out = sum;
   return WZFH_OK;
 }
@@ -321,6 +354,7 @@ int wzfh_serve_close(wzfh_server *S) {
   for (auto &t : S->stream_threads) t.join();
   for (auto &t : S->comp_threads) t.join();
   for (Group &G : S->groups) if (G.sh) wzfh_destroy(G.sh);
+  for (auto &v : S->comp_states) for (wzfh::DeviceState *d : v) wzfh::complete_state_free(d);
   const int rc = S->error;
   delete S;
   return rc;

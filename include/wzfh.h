@@ -238,16 +238,17 @@ int wzfh_set_tuning(wzfh_problem *p, const wzfh_tuning *t);
 
 /* ---- Serving: continuous batching of cell-search requests ------------------------
  * No reference counterpart (the reference runs fixed arm processes). Requests name a
- * problem (an initialized (n, class) handle, kept device-resident) and a cell; one worker
- * per problem admits whatever has arrived, up to max_batch cells, into one batched search
- * on the problem's own stream, so batches of different problems overlap on the device.
- * A result equals the same cell searched alone. Handles passed to a server must not be
- * used by the caller until wzfh_serve_close returns. */
+ * problem (an initialized (n, class) handle, kept device-resident) and a cell. The problems
+ * of one n share a stream stage that admits whatever has arrived, up to max_batch cells, into
+ * one persistent stream launch and hands each cell's top-K candidates to its problem's
+ * completion workers, which complete them on HIP streams of their own while the next launch
+ * streams; different n run concurrently. A result equals the same cell searched alone.
+ * Handles passed to a server must not be used by the caller until wzfh_serve_close returns. */
 typedef struct wzfh_server wzfh_server;
 typedef struct {
   wzfh_search_opts search;   /* per-cell unit (batched flag ignored: always batched) */
-  int max_batch;             /* cells admitted into one launch per problem            */
-  int reserved;
+  int max_batch;             /* cells admitted into one stream launch                 */
+  int complete_workers;      /* completion runs per problem at once (0: 1, at most 8)  */
 } wzfh_serve_opts;
 typedef struct {
   long long ticket;
@@ -335,7 +336,9 @@ long long wzfh_flat_score(int n, uint64_t C, uint64_t D);
  * 7 -- with WZFH_HALL_VERIFY=1, which also runs the FP64 test on every leaf -- leaves whose
  * certain decision disagreed with FP64 (a false reject or false accept; must be 0),
  * 8 donations, 9 donated work items, 11 lane DFS moves, 12 wave passes of the main loop.
- * kind 10: the leaf test's margin. */
+ * kind 10: the leaf test's margin. kind 13 (WZFH_TELEMETRY=1): completer nodes walked by all
+ * work items of the last completion run, including the partial walks of items stopped at the
+ * charge cap and walks past a budget abort (kind 1 counts each of the reference's nodes once). */
 int wzfh_last_kernel_ms(const wzfh_problem *p, int kind, double *ms, long long *units);
 /* Running totals of a handle since the last reset (or since wzfh_init), for measurement over
  * a timed region: summed HIP-event durations of the stream kernel's launches (index 0) and of

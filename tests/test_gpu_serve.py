@@ -1,5 +1,6 @@
 """Serving: requests of mixed n admitted by continuous batching give exactly the result of the
-same cell searched alone (one cell per launch), and every request is answered once."""
+same cell searched alone (one cell per launch), and every request is answered once, with one
+or several completion workers per problem."""
 import random
 import time
 
@@ -18,8 +19,8 @@ def key(r):
             r.aborted, r.A, r.B, r.C, r.D)
 
 
-@pytest.mark.parametrize('node_budget,cand_cap', [(200000, 0), (0, 64)])
-def test_served_results_equal_alone(node_budget, cand_cap):
+@pytest.mark.parametrize('node_budget,cand_cap,workers', [(200000, 0, 0), (0, 64, 0), (200000, 0, 1), (200000, 0, 3)])
+def test_served_results_equal_alone(node_budget, cand_cap, workers):
     probs = [wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1) for n, sig in SPEC]
     rng = random.Random(7)
     reqs = []
@@ -33,7 +34,8 @@ def test_served_results_equal_alone(node_budget, cand_cap):
     for pi, c in reqs:
         alone[(pi, c)] = key(probs[pi].search([c], top_k, budget, cand_cap=cand_cap, node_budget=node_budget,
                                              batched=False)[0])
-    S = wz.Server(probs, top_k, budget, cand_cap=cand_cap, node_budget=node_budget, max_batch=16)
+    S = wz.Server(probs, top_k, budget, cand_cap=cand_cap, node_budget=node_budget, max_batch=16,
+                  complete_workers=workers)
     try:
         tickets = {}
         for pi, c in reqs:
