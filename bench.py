@@ -559,9 +559,11 @@ def verify_sample(a, probs, reqs):
 
 def stream_utilization(P, cells, a):
     """One stream launch of a step-sized batch of one class with the kernel's counters on
-    (WZFH_TELEMETRY=1): DFS moves its lanes made, main-loop passes of its waves, and the share of
-    the moves that are counted nodes inside the cells' budgets (the rest is speculation past the
-    cuts, abandoned)."""
+    (WZFH_TELEMETRY=1): DFS moves its lanes made, main-loop passes of its waves, the share of the
+    lane slots of those passes that made a move, and counted nodes (inside the cells' budgets) per
+    move. A move enters one live child; the dead children it skips are counted nodes too (the
+    reference enters them and prunes), so a fully useful walk counts more than one node per move,
+    and moves beyond the cuts (speculation, abandoned) lower the ratio."""
     keep = os.environ.get('WZFH_TELEMETRY')
     os.environ['WZFH_TELEMETRY'] = '1'
     try:
@@ -578,7 +580,7 @@ def stream_utilization(P, cells, a):
     return {'cells': len(cells), 'launch_ms': round(ms, 2), 'lane_moves': moves, 'wave_passes': passes,
             'moves_per_wave_pass': round(moves / passes, 2) if passes else None,
             'lane_slot_use': round(moves / (passes * 64 * inner), 5) if passes else None,
-            'counted_share_of_moves': round(counted / moves, 4) if moves else None}
+            'counted_nodes_per_move': round(counted / moves, 4) if moves else None}
 
 
 def cpu_cores():
