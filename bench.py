@@ -16,8 +16,8 @@ every computation is the real one, results are exact.
 A step = one slice of that set: slice k holds cells j = k (mod 12) of every class's seeded
 sample (~1,024 cells across the 12 classes), so 12 consecutive steps search the whole
 12 x 1,024 set once; step s (warmup steps first) takes slice s mod 12. The steps are served:
-each step's cells are submitted to the continuous-batching server as one burst, with six steps
-in flight (--in-flight): step s+6 is submitted when every result of step s is back, so the
+each step's cells are submitted to the continuous-batching server as one burst, with eight steps
+in flight (--in-flight): step s+8 is submitted when every result of step s is back, so the
 stream launches of later steps run while earlier steps' completions finish, as under continuous
 load; a stream launch admits at most two steps' cells. The timed region starts with nothing in flight
 and ends when the last result of the last timed step is back. `value` is served cells/s over
@@ -216,7 +216,7 @@ def main():
     ap.add_argument('--latency-requests', type=int, default=1500)
     ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')
     ap.add_argument('--max-batch', type=int, default=0, help='cells per stream launch (0: two slices)')
-    ap.add_argument('--in-flight', type=int, default=6, help='served steps in flight at once')
+    ap.add_argument('--in-flight', type=int, default=8, help='served steps in flight at once')
     ap.add_argument('--serve-caps', default='11,12,13,14',
                     help='completer charge caps (log2) tried on served steps after autotuning (empty: skip)')
     ap.add_argument('--serve-tune-steps', type=int, default=4, help='served steps per cap trial')
@@ -286,8 +286,8 @@ def main():
         log(f'autotune: {tune}')
     log(f'setup: {len(cls)} classes, {sum(map(len, samples))} cells in {a.slices} slices, {setup_s:.1f} s')
 
-    # a stream launch admits at most two steps' cells (measured on the n = 44 set with 4-8 steps
-    # in flight: 5-10 % more cells/s than one step per launch, less with four)
+    # a stream launch admits at most two steps' cells (measured on the n = 44 set with 4-12 steps
+    # in flight: 5-10 % more cells/s than one step per launch; three or four steps per launch lose)
     max_batch = a.max_batch or 2 * max(len(sl) for sl in slices)
     if a.serve_caps and not a.tuning:
         tune = served_cap_tune(a, probs, slices, search, max_batch, tune)
