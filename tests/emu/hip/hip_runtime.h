@@ -33,7 +33,7 @@ inline EmuTid g_bdim{256, 1, 1}, g_gdim{1, 1, 1};
 #define __constant__ static
 #define __shared__ static thread_local_shared
 #define thread_local_shared
-#define __launch_bounds__(x)
+#define __launch_bounds__(...)
 typedef int hipError_t;
 typedef void *hipStream_t;
 typedef double *hipEvent_t;
