@@ -68,21 +68,34 @@ std::string set_error(const std::string &msg) { g_err = msg; return msg; }
     }                                                                                  \
   } while (0)
 
-// Growable device buffer.
+// A device buffer that only grows. It grows by at least half its size (rounded up to 2 MiB above
+// 2 MiB), so a server whose launches vary in size reallocates rarely, and the buffer it outgrows
+// is kept until release(): hipFree waits for the device to finish its queued work, which would
+// stall every other stream's pipeline in the middle of serving (the kept buffers add at most
+// twice the final size).
 struct DBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  std::vector<void *> retired;
   int ensure(size_t want) {
     if (want <= bytes) return WZFH_OK;
-    if (p) (void)hipFree(p);
+    if (p) retired.push_back(p);
     p = nullptr;
-    bytes = 0;
     size_t b = std::max<size_t>(want, 256);
+    if (bytes > 0) b = std::max(b, bytes + bytes / 2);
+    if (b > (2u << 20)) b = (b + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+    bytes = 0;
     if (hipMalloc(&p, b) != hipSuccess) { p = nullptr; set_error("hipMalloc failed"); return WZFH_ENOMEM; }
     bytes = b;
     return WZFH_OK;
   }
-  void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+  void release() {
+    if (p) (void)hipFree(p);
+    for (void *q : retired) (void)hipFree(q);
+    retired.clear();
+    p = nullptr;
+    bytes = 0;
+  }
   template <typename T> T *as() const { return (T *)p; }
 };
 
@@ -1035,26 +1048,18 @@ struct Frame {
 };
 }  // namespace
 
-int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
-                 wzfh_completion *out, Carve *carve, long long *depth_nodes, DeviceState *dev) {
-  DeviceState *S = dev ? dev : P.dev;
-  if (ncand <= 0) return WZFH_OK;
+// The rounds of one completer run over the candidates cdv / cellv (packed (C, D) pairs and cell
+// indices, -1 = no profile rows). With a feed, candidates also join between rounds (feed->more
+// appends them) and each is reported through feed->done as soon as it is decided; fin holds every
+// candidate's result at the end.
+static int complete_rounds(wzfh_problem &P, std::vector<uint64_t> &cdv, std::vector<int> &cellv, long long budget,
+                           Carve *carve, long long *depth_nodes, DeviceState *S, CompleteFeed *feed,
+                           std::vector<Result> &fin) {
   WZ_HIP(hipSetDevice(S->device));
-  std::vector<int> roff(ncand, -1), rlen(ncand, 0);
+  std::vector<int> roff, rlen;
   int maxrows = 0;
-  for (int i = 0; i < ncand; i++) {
-    const int c = cell_of ? cell_of[i] : -1;
-    if (c < 0 || !P.ab_prof) continue;
-    if (c >= (int)P.cells.size()) { set_error("cell index out of range"); return WZFH_EINVAL; }
-    const int li = P.cell_rows[c];
-    if (li < 0) { roff[i] = 0; rlen[i] = 0; continue; }   // proven-dead cell: no row can complete
-    if ((long long)P.row_len[li] > P.cfg.abp_cap) continue;   // oversized list: unconstrained
-    roff[i] = P.row_off[li];
-    rlen[i] = P.row_len[li];
-    maxrows = std::max(maxrows, rlen[i]);
-  }
   const bool limited = budget > 0;
-  std::vector<Result> fin(ncand, Result{2, 0, 0, 0, 0, 0, false});
+  fin.clear();
   S->walked = 0;
   double ms = 0;
   // per-depth counts need whole walks: one item per candidate
@@ -1064,7 +1069,27 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
   const long long cap = ext0 > 0 ? item_charge_cap(S) : 0;
   std::vector<Frame> frames;
   std::vector<Job> jobs, next_jobs;
-  for (int i = 0; i < ncand; i++) jobs.push_back(Job{i, 0, 0, budget, 0, 0, -1, i, ext0 == 0});
+  // candidates [from, cellv.size()) join the run: their rows and their first-round jobs
+  auto add = [&](size_t from) -> int {
+    for (size_t i = from; i < cellv.size(); i++) {
+      int ro = -1, rl = 0;
+      const int c = cellv[i];
+      if (c >= 0 && P.ab_prof) {
+        if (c >= (int)P.cell_rows.size()) { set_error("cell index out of range"); return WZFH_EINVAL; }
+        const int li = P.cell_rows[c];
+        if (li < 0) { ro = 0; rl = 0; }   // proven-dead cell: no row can complete
+        else if ((long long)P.row_len[li] <= P.cfg.abp_cap) { ro = P.row_off[li]; rl = P.row_len[li]; }
+        // else: oversized list, unconstrained
+      }
+      roff.push_back(ro);
+      rlen.push_back(rl);
+      maxrows = std::max(maxrows, rl);
+      fin.push_back(Result{2, 0, 0, 0, 0, 0, false});
+      jobs.push_back(Job{(int)i, 0, 0, budget, 0, 0, -1, (int)i, ext0 == 0});
+    }
+    return WZFH_OK;
+  };
+  if (int rc = add(0)) return rc;
 
   std::function<void(int, int, Result)> deliver;
   // scans frame f from its next item; returns when it waits for a capped item or is closed
@@ -1113,13 +1138,36 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
   };
   deliver = [&](int owner, int slot, Result r) {
     r.done = true;
-    if (owner < 0) { fin[slot] = r; return; }
+    if (owner < 0) {
+      fin[slot] = r;
+      {
+        std::lock_guard<std::mutex> lk(S->tot_mu);
+        S->tot_units[1] += r.n;
+      }
+      if (feed) {
+        wzfh_completion c{};
+        c.verdict = r.v;
+        c.nodes = r.n;
+        c.A = r.v == 0 || r.v == 4 ? r.A : 0;
+        c.B = r.v == 0 || r.v == 4 ? r.B : 0;
+        feed->done(slot, c);
+      }
+      return;
+    }
     frames[owner].subs[slot] = r;
     resume(owner);
   };
 
   int round = 0;
-  while (!jobs.empty()) {
+  while (true) {
+    if (feed) {   // candidates that arrived since the last round join this one
+      const size_t before = cellv.size();
+      if (feed->more(cdv, cellv) > 0) {
+        if (int rc = add(before)) return rc;
+        round = 0;
+      }
+    }
+    if (jobs.empty()) break;
     if (++round > 4096) { set_error("completer: rounds did not converge (internal error)"); return WZFH_EKERNEL; }
     next_jobs.clear();
     // launches: exact walks, first-round splits (root canon pair), deeper splits
@@ -1144,7 +1192,7 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
         std::vector<uint64_t> A(ne), B(ne);
         std::vector<std::vector<ItemRes>> items;
         std::vector<long long> dep;
-        int rc = complete_launch(P, S, cd, roff.data(), rlen.data(), part, maxrows, budget, ext, icap, v.data(),
+        int rc = complete_launch(P, S, cdv.data(), roff.data(), rlen.data(), part, maxrows, budget, ext, icap, v.data(),
                                  nd.data(), A.data(), B.data(), j.data(), cc.data(), cj.data(), &ms, carve, &items,
                                  depth_nodes ? &dep : nullptr);
         if (rc != WZFH_OK) return rc;
@@ -1200,24 +1248,42 @@ int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int nc
   S->last_ms[1] = ms;
   S->last_walked = S->walked;
   long long units = 0;
+  for (const Result &r : fin) {
+    if (!r.done) { set_error("completer: a candidate was left undecided (internal error)"); return WZFH_EKERNEL; }
+    units += r.n;
+  }
+  S->last_units[1] = units;
+  return WZFH_OK;
+}
+
+int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
+                 wzfh_completion *out, Carve *carve, long long *depth_nodes, DeviceState *dev) {
+  DeviceState *S = dev ? dev : P.dev;
+  if (ncand <= 0) return WZFH_OK;
+  std::vector<uint64_t> cdv(cd, cd + 2 * (size_t)ncand);
+  std::vector<int> cellv(ncand, -1);
+  if (cell_of) std::copy(cell_of, cell_of + ncand, cellv.begin());
+  std::vector<Result> fin;
+  if (int rc = complete_rounds(P, cdv, cellv, budget, carve, depth_nodes, S, nullptr, fin)) return rc;
   bool internal = false;
   for (int i = 0; i < ncand; i++) {
-    if (!fin[i].done) { set_error("completer: a candidate was left undecided (internal error)"); return WZFH_EKERNEL; }
     out[i].verdict = fin[i].v;
     out[i].reserved = 0;
     out[i].nodes = fin[i].n;
     out[i].A = fin[i].v == 0 || fin[i].v == 4 ? fin[i].A : 0;
     out[i].B = fin[i].v == 0 || fin[i].v == 4 ? fin[i].B : 0;
-    units += fin[i].n;
     if (fin[i].v == 4) internal = true;
-  }
-  S->last_units[1] = units;
-  {
-    std::lock_guard<std::mutex> lk(S->tot_mu);
-    S->tot_units[1] += units;
   }
   if (internal) { set_error("completer FOUND failed the exact NPAF recheck"); return WZFH_EINTERNAL; }
   return WZFH_OK;
+}
+
+int run_complete_feed(wzfh_problem &P, long long budget, DeviceState *dev, CompleteFeed &feed) {
+  DeviceState *S = dev ? dev : P.dev;
+  std::vector<uint64_t> cdv;
+  std::vector<int> cellv;
+  std::vector<Result> fin;
+  return complete_rounds(P, cdv, cellv, budget, nullptr, nullptr, S, &feed, fin);
 }
 
 // Per cell of a stream batch: stable-sort its candidates by flat score and take the first K

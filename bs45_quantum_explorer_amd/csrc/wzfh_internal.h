@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -96,6 +97,16 @@ struct Carve;
 int run_complete(wzfh_problem &P, const uint64_t *cd, const int *cell_of, int ncand, long long budget,
                  wzfh_completion *out, Carve *carve = nullptr, long long *depth_nodes = nullptr,
                  DeviceState *dev = nullptr);
+// A completer run that candidates join while it runs (the serving loop's continuous batching):
+// before every round more() appends the packed (C, D) pairs and cell indices of the candidates
+// that arrived and returns how many; done(i, c) reports candidate i (in joining order) as soon as
+// it is decided. The run ends when no candidate is left and more() adds none. A candidate's
+// result does not depend on what else shares the run.
+struct CompleteFeed {
+  std::function<int(std::vector<uint64_t> &cd, std::vector<int> &cell_of)> more;
+  std::function<void(int, const wzfh_completion &)> done;
+};
+int run_complete_feed(wzfh_problem &P, long long budget, DeviceState *dev, CompleteFeed &feed);
 // A device state for completer runs of P only (own HIP stream and buffers, copies of P's
 // completer tables), so several completion runs of one problem can run side by side.
 int complete_state(const wzfh_problem &P, DeviceState **out);

@@ -11,12 +11,12 @@
 // cells that fit the group's share of the device's free memory, re-read at every admission --
 // runs them as ONE persistent stream launch (cells of different classes side by side), picks
 // each cell's flattest top_k candidates, hands them to the problems' completion stages and
-// immediately admits the next requests. Each problem has a completion stage: a thread that
-// takes every stream batch queued for it and completes their candidates in one completer run
-// under the cells' profile rows on the problem's own HIP stream, then returns the results.
-// So a request that arrives while its group is streaming waits for that one launch only; the
-// completions of earlier batches run meanwhile on the other streams, and groups of different n
-// run concurrently. A result equals the same cell searched alone: a cell's stream is exact per
+// immediately admits the next requests. Each problem has a completion stage: a thread running
+// streaming completer runs under the cells' profile rows on the problem's own HIP stream; the
+// stream batches queued for the problem join the running run between its rounds, and each
+// batch's results return as soon as its own candidates are decided. So a request that arrives
+// while its group is streaming waits for that one launch only; the completions of earlier
+// batches run meanwhile on the other streams, and groups of different n run concurrently. A result equals the same cell searched alone: a cell's stream is exact per
 // cell whatever else shares its launch, and its completions depend only on its candidates.
 #include <algorithm>
 #include <chrono>
@@ -162,62 +162,96 @@ static void stream_loop(wzfh_server *S, int g) {
 }
 
 // Completion worker w of problem pidx: worker 0 runs on the problem's own device state, the
-// others on states of their own, so a batch that arrives while a run waits for its longest
-// items starts at once on another worker.
+// others on states of their own. A worker runs one streaming completer run at a time: the
+// batches queued for its problem join the run between its rounds (continuous batching inside the
+// completer), and each batch's results go out as soon as its own candidates are decided. A batch
+// that arrives while a run waits for its heaviest candidates therefore starts at the next round
+// instead of after the whole run.
 static void complete_loop(wzfh_server *S, int pidx, int w) {
   wzfh_problem *P = S->problems[pidx];
   wzfh::DeviceState *dev = w > 0 ? S->comp_states[pidx][w - 1] : nullptr;
   const int g = S->group_of[pidx];
+  constexpr long long kRunCands = 1 << 20;   // candidates one run takes before it lets a new run start
   while (true) {
-    std::vector<CompBatch> batches;
     {
       std::unique_lock<std::mutex> lk(S->mu);
       S->cv_comp.wait(lk, [&] { return !S->comp_q[pidx].empty() || S->groups[g].stream_done; });
       if (S->comp_q[pidx].empty()) return;
-      while (!S->comp_q[pidx].empty()) {
-        batches.push_back(std::move(S->comp_q[pidx].front()));
-        S->comp_q[pidx].pop_front();
-      }
     }
-    // every queued batch's candidates in one completer run
-    std::vector<uint64_t> cd;
-    std::vector<int> cell_of;
-    for (const CompBatch &b : batches) {
-      cd.insert(cd.end(), b.topk.cd.begin(), b.topk.cd.end());
-      for (int o : b.topk.owner) cell_of.push_back(b.reqs[o].cell);
-    }
-    const int nk = (int)cell_of.size();
-    std::vector<wzfh_completion> comp(nk);
-    if (dev) wzfh::complete_state_sync_tuning(*P, dev);
-    int rc = nk ? wzfh::run_complete(*P, cd.data(), cell_of.data(), nk, S->opts.search.budget, comp.data(), nullptr,
-                                     nullptr, dev)
-                : WZFH_OK;
-    const std::string msg = rc != WZFH_OK ? wzfh_last_error() : "";
-    const auto finished = Clock::now();
-    std::lock_guard<std::mutex> lk(S->mu);
-    if (rc != WZFH_OK && rc != WZFH_EINTERNAL) {
-      for (const CompBatch &b : batches) S->fail(b.reqs, rc, msg.c_str(), b.started, b.launch_cells);
-      continue;
-    }
-    size_t at = 0;
-    for (CompBatch &b : batches) {
-      wzfh::aggregate_completions(b.topk, comp.data() + at, b.res.data());
-      at += b.topk.owner.size();
-      for (size_t i = 0; i < b.reqs.size(); i++) {
+    struct Live {
+      CompBatch b;
+      int base = 0, left = 0;
+      bool out = false;
+    };
+    std::deque<Live> live;        // batches of this run, in joining order (stable references)
+    std::vector<int> live_of;     // candidate -> batch
+    std::vector<wzfh_completion> comp;
+    long long taken = 0;
+    auto finish = [&](Live &L) {   // every candidate of the batch is decided: its results go out
+      wzfh::aggregate_completions(L.b.topk, comp.data() + L.base, L.b.res.data());
+      const auto finished = Clock::now();
+      std::lock_guard<std::mutex> lk(S->mu);
+      for (size_t i = 0; i < L.b.reqs.size(); i++) {
         wzfh_serve_result r{};
-        r.ticket = b.reqs[i].ticket;
+        r.ticket = L.b.reqs[i].ticket;
         r.problem = pidx;
-        r.cell = b.reqs[i].cell;
-        r.status = (rc == WZFH_EINTERNAL && b.res[i].internal) ? WZFH_EINTERNAL : WZFH_OK;
-        r.batch_size = b.launch_cells;
-        r.result = b.res[i];
-        r.submit_s = S->since(b.reqs[i].submitted);
-        r.start_s = S->since(b.started);
+        r.cell = L.b.reqs[i].cell;
+        r.status = L.b.res[i].internal ? WZFH_EINTERNAL : WZFH_OK;
+        r.batch_size = L.b.launch_cells;
+        r.result = L.b.res[i];
+        r.submit_s = S->since(L.b.reqs[i].submitted);
+        r.start_s = S->since(L.b.started);
         r.finish_s = S->since(finished);
         S->done.push_back(r);
       }
+      L.out = true;
+      S->cv_done.notify_all();
+    };
+    wzfh::CompleteFeed feed;
+    feed.more = [&](std::vector<uint64_t> &cd, std::vector<int> &cell_of) -> int {
+      if (taken >= kRunCands) return 0;
+      std::vector<CompBatch> got;
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        while (!S->comp_q[pidx].empty()) {
+          got.push_back(std::move(S->comp_q[pidx].front()));
+          S->comp_q[pidx].pop_front();
+        }
+      }
+      int added = 0;
+      for (CompBatch &b : got) {
+        live.emplace_back();
+        Live &L = live.back();
+        L.b = std::move(b);
+        L.base = (int)cell_of.size();
+        L.left = (int)L.b.topk.owner.size();
+        const int li = (int)live.size() - 1;
+        for (size_t k = 0; k < L.b.topk.owner.size(); k++) {
+          cd.push_back(L.b.topk.cd[2 * k]);
+          cd.push_back(L.b.topk.cd[2 * k + 1]);
+          cell_of.push_back(L.b.reqs[L.b.topk.owner[k]].cell);
+          live_of.push_back(li);
+          comp.emplace_back();
+        }
+        added += L.left;
+        taken += L.left;
+        if (L.left == 0) finish(L);   // no candidate in any of its cells
+      }
+      return added;
+    };
+    feed.done = [&](int i, const wzfh_completion &c) {
+      comp[i] = c;
+      Live &L = live[live_of[i]];
+      if (--L.left == 0) finish(L);
+    };
+    if (dev) wzfh::complete_state_sync_tuning(*P, dev);
+    const int rc = wzfh::run_complete_feed(*P, S->opts.search.budget, dev, feed);
+    if (rc != WZFH_OK) {
+      const std::string msg = wzfh_last_error();
+      std::lock_guard<std::mutex> lk(S->mu);
+      for (Live &L : live)
+        if (!L.out) S->fail(L.b.reqs, rc, msg.c_str(), L.b.started, L.b.launch_cells);
     }
-    S->cv_done.notify_all();
   }
 }
 
