@@ -1171,10 +1171,16 @@ static int complete_rounds(wzfh_problem &P, std::vector<uint64_t> &cdv, std::vec
     if (++round > 4096) { set_error("completer: rounds did not converge (internal error)"); return WZFH_EKERNEL; }
     next_jobs.clear();
     // launches: exact walks, first-round splits (root canon pair), deeper splits
+    // A run that candidates join (feed) launches first-round and deeper split jobs together: a
+    // launch then enumerates every depth-0 digit, and under the root canon the six digits the
+    // canon rejects end at once with no charge, so the results are the same and a round waits
+    // for one split launch instead of two.
+    const bool merge_splits = feed != nullptr;
     for (int kind = 0; kind < 3; kind++) {
+      if (merge_splits && kind == 2) break;
       std::vector<Job> group;
       for (const Job &x : jobs)
-        if ((kind == 0) == x.whole && (kind == 0 || (kind == 1) == (x.base == 0))) group.push_back(x);
+        if ((kind == 0) == x.whole && (kind == 0 || merge_splits || (kind == 1) == (x.base == 0))) group.push_back(x);
       if (group.empty()) continue;
       const int ext = kind == 0 ? 0 : ext0;
       const long long icap = kind == 0 ? 0 : cap;
@@ -1200,7 +1206,9 @@ static int complete_rounds(wzfh_problem &P, std::vector<uint64_t> &cdv, std::vec
           for (int e = 0; e < ne; e++)
             std::copy(dep.begin() + (size_t)e * 64, dep.begin() + (size_t)e * 64 + 64,
                       depth_nodes + (size_t)group[c0 + e].cand * 64);
-        const bool root_pair = ext > 0 && !P.cfg.no_canon_ab && kind == 1;
+        // as complete_launch decides it: only a launch of root entries alone takes the root pair
+        bool root_pair = ext > 0 && !P.cfg.no_canon_ab;
+        for (const CompleteEntry &x : part) root_pair = root_pair && x.base == 0;
         for (int e = 0; e < ne; e++) {
           const Job &x = group[c0 + e];
           if (v[e] >= 0) {   // decided: hit, clean, pre-rejected, or an exactly counted abort
