@@ -4,6 +4,7 @@ tests/test_emu_kernels.py so the emulated library never mixes with the real one.
 import hashlib
 import os
 import sys
+import time
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -18,10 +19,13 @@ from golden_data import complete_cases, stream_runs  # noqa: E402
 bad = 0
 
 
+T0 = time.time()
+
+
 def report(ok, *msg):
     global bad
     bad += 0 if ok else 1
-    print('ok ' if ok else 'BAD', *msg, flush=True)
+    print('ok ' if ok else 'BAD', f'[{time.time() - T0:6.1f} s]', *msg, flush=True)
 
 
 # completer: budget aborts found through split items, capped items re-split deeper, the
@@ -30,10 +34,14 @@ cases = [c for c in complete_cases() if c['n'] == 13 and c['name'] in ('n13:5', 
 P = wz.Problem(13, tuple(cases[0]['sig']), m6=0, thm211b=0, thm212=0, outer_k=cases[0]['outer_k'])
 os.environ['WZFH_CHECK_REDUCE'] = '1'
 os.environ['WZFH_COMPLETE_EXACT_ITEM'] = '1'
-plan = [(2, 0, c) for c in cases] + [(1, 4, c) for c in cases if (c['name'], c['budget']) in (('n13:5', 0), ('n13:23', 300))]
+# capped items (charge cap 64 under budget 300: the abort falls inside a capped item's subtree)
+# walk abort items of up to 2,048 charges whole; a smaller cap or exact-item threshold multiplies
+# the emulated launches (cap 4: ~200 s on this emulation)
+plan = [(2, 0, c) for c in cases] + [(1, 64, c) for c in cases if (c['name'], c['budget']) in (('n13:5', 0), ('n13:23', 300))]
 for split, cap, c in plan:
     os.environ['WZFH_COMPLETE_SPLIT'] = str(split)
     os.environ['WZFH_COMPLETE_ITEM_CAP'] = str(cap)
+    os.environ['WZFH_COMPLETE_EXACT_ITEM'] = '1' if cap == 0 else '2048'
     if True:
         cd = np.array([[wz.pack(c['C']), wz.pack(c['D'])]], dtype=np.uint64)
         o = P.complete(cd, budget=c['budget'])[0]
@@ -46,7 +54,8 @@ for split, cap, c in plan:
 # library's static scratch array (two calls in a row, the second smaller)
 os.environ['WZFH_COMPLETE_SPLIT'] = '2'
 os.environ['WZFH_COMPLETE_ITEM_CAP'] = '0'
-for batch in (cases, cases[:2]):
+os.environ['WZFH_COMPLETE_EXACT_ITEM'] = '1'
+for batch in (cases[:2], cases[:1]):
     cd = np.array([[wz.pack(c['C']), wz.pack(c['D'])] for c in batch], dtype=np.uint64)
     for budget in sorted({c['budget'] for c in batch}):
         mine = [c for c in batch if c['budget'] == budget]
