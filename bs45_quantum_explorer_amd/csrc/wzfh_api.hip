@@ -117,7 +117,7 @@ struct DeviceState {
   DBuf it_nodes, it_noff, it_loff, it_coff, it_leaves, it_cands, it_state, it_next, it_key, it_meta, it_cell, it_tail;
   DBuf dq, cell_lock, cell_crossed, cell_wp, cell_pn, cell_pl, cell_pc, cell_open, cell_next, cell_sp;
   uint32_t epoch = 0;              // launch epoch tagging record states
-  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_icharged, c_ebase, c_epath, c_ebud,
+  DBuf c_cd, c_roff, c_rlen, c_verdict, c_nodes, c_A, c_B, c_charges, c_icharged, c_ebase, c_epath, c_ebud, c_eskip,
        c_fv, c_fn, c_fA, c_fB, c_fj, c_fcc, c_fcur, c_scratch, c_depth;
   double last_ms[2] = {0, 0};
   long long last_units[2] = {0, 0};
@@ -362,7 +362,7 @@ static void free_state(DeviceState *S) {
                   &S->it_next, &S->it_key, &S->it_meta, &S->it_cell, &S->it_tail, &S->dq, &S->cell_lock,
                   &S->cell_crossed, &S->cell_wp, &S->cell_pn, &S->cell_pl, &S->cell_pc, &S->cell_open,
                   &S->cell_next, &S->cell_sp, &S->c_cd, &S->c_roff, &S->c_rlen, &S->c_verdict, &S->c_nodes, &S->c_A,
-                  &S->c_B, &S->c_charges, &S->c_icharged, &S->c_ebase, &S->c_epath, &S->c_ebud,
+                  &S->c_B, &S->c_charges, &S->c_icharged, &S->c_ebase, &S->c_epath, &S->c_ebud, &S->c_eskip,
                   &S->c_fv, &S->c_fn, &S->c_fA, &S->c_fB, &S->c_fj, &S->c_fcc, &S->c_fcur, &S->c_scratch,
                   &S->c_depth})
     b->release();
@@ -697,13 +697,14 @@ struct CompleteEntry {
   int base;              // depth of the subtree root
   long long budget;      // charges left to the candidate's budget at the subtree start
   long long pre_nodes;   // counted nodes of the candidate before the subtree
+  int skip = 0;          // items below this were walked by the capped item the subtree continues
 };
 
 // Device buffers of one completer launch: per entry (inputs, reduce outputs), per work item
 // (results, charges) and the per-wave row scratch. They come either from the handle's growable
 // buffers or, for the allocation-free entry point, from a carve of this device's `scratch`.
 struct CompleteBufs {
-  uint64_t *cd = nullptr; int *roff = nullptr, *rlen = nullptr, *ebase = nullptr;
+  uint64_t *cd = nullptr; int *roff = nullptr, *rlen = nullptr, *ebase = nullptr, *eskip = nullptr;
   long long *epath = nullptr, *ebud = nullptr;
   int *verdict = nullptr; long long *nodes = nullptr; uint64_t *A = nullptr, *B = nullptr;
   long long *charges = nullptr, *icharged = nullptr;
@@ -726,14 +727,15 @@ struct Carve {
 
 // Per-item and per-entry bytes of a launch in a carve (with the alignment of each piece).
 constexpr size_t kItemBytes = 4 + 8 + 8 + 8 + 8 + 8;
-constexpr size_t kEntryBytes = 16 + 4 + 4 + 4 + 8 + 8 + 4 + 8 + 8 + 8 + 4 + 8 + 8;
-constexpr size_t kCarveSlack = 24 * 256 + 64;
+constexpr size_t kEntryBytes = 16 + 4 + 4 + 4 + 4 + 8 + 8 + 4 + 8 + 8 + 8 + 4 + 8 + 8;
+constexpr size_t kCarveSlack = 25 * 256 + 64;
 
 static int complete_buffers(DeviceState *S, Carve *carve, int nent, size_t nw, size_t row_scratch,
                             CompleteBufs &b) {
   if (!carve) {
     if (S->c_cd.ensure((size_t)nent * 16) || S->c_roff.ensure(nent * 4) || S->c_rlen.ensure(nent * 4) ||
         S->c_ebase.ensure(nent * 4) || S->c_epath.ensure(nent * 8) || S->c_ebud.ensure(nent * 8) ||
+        S->c_eskip.ensure(nent * 4) ||
         S->c_verdict.ensure(nw * 4) || S->c_nodes.ensure(nw * 8) || S->c_A.ensure(nw * 8) || S->c_B.ensure(nw * 8) ||
         S->c_charges.ensure(nw * 8) || S->c_icharged.ensure(nw * 8) || S->c_scratch.ensure(row_scratch) ||
         S->ctr.ensure(64) || S->c_fv.ensure(nent * 4) || S->c_fn.ensure(nent * 8) || S->c_fA.ensure(nent * 8) ||
@@ -742,6 +744,7 @@ static int complete_buffers(DeviceState *S, Carve *carve, int nent, size_t nw, s
       return WZFH_ENOMEM;
     b.cd = S->c_cd.as<uint64_t>(); b.roff = S->c_roff.as<int>(); b.rlen = S->c_rlen.as<int>();
     b.ebase = S->c_ebase.as<int>(); b.epath = S->c_epath.as<long long>(); b.ebud = S->c_ebud.as<long long>();
+    b.eskip = S->c_eskip.as<int>();
     b.verdict = S->c_verdict.as<int>(); b.nodes = S->c_nodes.as<long long>();
     b.A = S->c_A.as<uint64_t>(); b.B = S->c_B.as<uint64_t>();
     b.charges = S->c_charges.as<long long>(); b.icharged = S->c_icharged.as<long long>();
@@ -756,6 +759,7 @@ static int complete_buffers(DeviceState *S, Carve *carve, int nent, size_t nw, s
   b.cd = carve->take<uint64_t>((size_t)nent * 16); b.roff = carve->take<int>(nent * 4);
   b.rlen = carve->take<int>(nent * 4); b.ebase = carve->take<int>(nent * 4);
   b.epath = carve->take<long long>(nent * 8); b.ebud = carve->take<long long>(nent * 8);
+  b.eskip = carve->take<int>(nent * 4);
   b.verdict = carve->take<int>(nw * 4); b.nodes = carve->take<long long>(nw * 8);
   b.A = carve->take<uint64_t>(nw * 8); b.B = carve->take<uint64_t>(nw * 8);
   b.charges = carve->take<long long>(nw * 8); b.icharged = carve->take<long long>(nw * 8);
@@ -763,7 +767,7 @@ static int complete_buffers(DeviceState *S, Carve *carve, int nent, size_t nw, s
   b.fB = carve->take<uint64_t>(nent * 8); b.fj = carve->take<int>(nent * 4); b.fcc = carve->take<long long>(nent * 8);
   b.fcur = carve->take<long long>(nent * 8);
   b.row_scratch = carve->take<uint64_t>(row_scratch);
-  if (!b.ctr || !b.cd || !b.roff || !b.rlen || !b.ebase || !b.epath || !b.ebud || !b.verdict || !b.nodes || !b.A ||
+  if (!b.ctr || !b.cd || !b.roff || !b.rlen || !b.ebase || !b.eskip || !b.epath || !b.ebud || !b.verdict || !b.nodes || !b.A ||
       !b.B || !b.charges || !b.icharged || !b.fv || !b.fn || !b.fA || !b.fB || !b.fj || !b.fcc || !b.fcur ||
       !b.row_scratch) {
     set_error("allocation-free completer: a launch does not fit the static scratch array");
@@ -802,11 +806,15 @@ static int complete_launch(wzfh_problem &P, DeviceState *S, const uint64_t *cd, 
   std::vector<uint64_t> ecd(2 * (size_t)nent);
   std::vector<int> ero(nent), erl(nent), ebase(nent);
   std::vector<long long> epath(nent), ebud(nent);
+  std::vector<int> eskip(nent);
+  bool any_skip = false;
   for (int e = 0; e < nent; e++) {
     const CompleteEntry &x = ents[e];
     ecd[2 * e] = cd[2 * (size_t)x.cand]; ecd[2 * e + 1] = cd[2 * (size_t)x.cand + 1];
     ero[e] = roff[x.cand]; erl[e] = rlen[x.cand];
     ebase[e] = x.base; epath[e] = x.path; ebud[e] = x.budget;
+    eskip[e] = x.skip;
+    any_skip = any_skip || x.skip > 0;
   }
   CompleteBufs bf;
   if (int rc = complete_buffers(S, carve, nent, nw, row_scratch, bf)) return rc;
@@ -816,6 +824,7 @@ static int complete_launch(wzfh_problem &P, DeviceState *S, const uint64_t *cd, 
   WZ_HIP(hipMemcpyAsync(bf.ebase, ebase.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
   WZ_HIP(hipMemcpyAsync(bf.epath, epath.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
   WZ_HIP(hipMemcpyAsync(bf.ebud, ebud.data(), nent * 8, hipMemcpyHostToDevice, S->stream));
+  if (any_skip) WZ_HIP(hipMemcpyAsync(bf.eskip, eskip.data(), nent * 4, hipMemcpyHostToDevice, S->stream));
   // counters start at zero (a fill kernel, not a memset node: callers of the allocation-free
   // entry point may record this stream into a graph)
   WZ_HIP(launch_zero(bf.ctr, 64, S->stream));
@@ -828,6 +837,8 @@ static int complete_launch(wzfh_problem &P, DeviceState *S, const uint64_t *cd, 
   a.split_ext = ext; a.nwork = nwork; a.root_pair = root_pair ? 1 : 0; a.items_log2 = items_log2;
   a.item_cap = ext > 0 ? item_cap : 0;
   a.ent_path = bf.epath; a.ent_base = bf.ebase; a.ent_budget = bf.ebud;
+  a.ent_skip = any_skip ? bf.eskip : nullptr;
+  a.snap_ext = (ext > 0 && item_cap > 0) ? ext : 0;   // capped items report where they stopped
   a.cd = bf.cd;
   a.row_off = bf.roff; a.row_len = bf.rlen;
   a.rows = S->rows.p ? S->rows.as<uint64_t>() : bf.cd;
@@ -1034,6 +1045,7 @@ struct Job {
   long long pre_n, pre_c;    // nodes / charges of the target before this job
   int owner, slot;           // target: frame `owner`'s capped-item slot, or the candidate (-1)
   bool whole;                // walk as one item (an exactly counted abort)
+  int skip = 0;              // split items below this were walked by the capped item it continues
 };
 struct Frame {
   int cand; long long path; int base, ext; bool root_pair;
@@ -1191,7 +1203,7 @@ static int complete_rounds(wzfh_problem &P, std::vector<uint64_t> &cdv, std::vec
       for (size_t c0 = 0; c0 < group.size(); c0 += chunk) {
         std::vector<CompleteEntry> part;
         for (size_t e = c0; e < std::min(group.size(), c0 + chunk); e++)
-          part.push_back(CompleteEntry{group[e].cand, group[e].path, group[e].base, group[e].budget, 0});
+          part.push_back(CompleteEntry{group[e].cand, group[e].path, group[e].base, group[e].budget, 0, group[e].skip});
         const int ne = (int)part.size();
         std::vector<int> v(ne), j(ne);
         std::vector<long long> nd(ne), cc(ne), cj(ne);
@@ -1240,6 +1252,17 @@ static int complete_rounds(wzfh_problem &P, std::vector<uint64_t> &cdv, std::vec
                                               : (long long)i;
               Job y{x.cand, (x.path << (3 * ext)) | dig, x.base + ext, x.budget - lower, 0, 0, f, (int)F.subs.size(), false};
               y.whole = y.base + ext0 > max_base;
+              if (!y.whole && ext0 == ext) {
+                // the capped item's walk covered its subtree's sub-paths below the one it stopped
+                // in: the next round splits the subtree from that sub-path on, and the charges and
+                // nodes counted before it are carried as the job's start
+                const long long sc = (long long)(it.A & ((1ull << 44) - 1));
+                const int sj = (int)(it.A >> 44);
+                y.skip = sj;
+                y.pre_c = sc;
+                y.pre_n = (long long)it.B;
+                y.budget -= sc;
+              }
               F.slot_of[i] = (int)F.subs.size();
               F.subs.push_back(Result{0, 0, 0, 0, 0, 0, false});
               next_jobs.push_back(y);

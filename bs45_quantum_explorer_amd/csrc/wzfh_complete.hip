@@ -141,6 +141,14 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
     // other depth-0 quad is cut before it is charged, so those paths carry no work)
     const int psub = a.root_pair ? (((6 | (sub >> (3 * (ext - 1)))) << (3 * (ext - 1))) | (sub & ((1 << (3 * (ext - 1))) - 1)))
                                  : sub;
+    if (a.ent_skip && sub < uni(a.ent_skip[ci])) {   // walked already (see CompleteArgs::ent_skip)
+      a.verdict[u] = 2;
+      a.nodes[u] = 0;
+      a.outA[u] = 0;
+      a.outB[u] = 0;
+      a.charges[u] = 0;
+      continue;
+    }
     const int base = uni((a.ent_base ? a.ent_base[ci] : 0) + ext);   // levels replayed along the path
     const long long path = uni64(((a.ent_path ? a.ent_path[ci] : 0ll) << (3 * ext)) | psub);   // depth l: bits 3(base-1-l)
     // a budgeted launch (a.budget > 0) gives each entry the charges left to it, possibly 0
@@ -158,6 +166,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
     int verdict = 2;
     long long total = 0, cur = 0;   // nodes counted; nodes charged to the budget
     long long dn = 0;               // lane l: nodes counted at depth l (FH_TELEM depth_nodes)
+    long long snap_c = 0, snap_n = 0;   // charges (| sub-path << 44) and nodes before the current sub-path
     uint64_t A = 0, B = 0;
     // pre-filter (wz_match.cpp:1305-1308): |T[s]| > 2 (L - s) for some s < L
     const bool prerej = any_lane(valid && (T > 2 * (L - s) || -T > 2 * (L - s)));
@@ -386,6 +395,19 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
             if (rest == 0) { state = S_RETURN; continue; }
             k = __ffs((int)rest) - 1;
             kcur = k;
+            if (d < base + a.snap_ext) {
+              // the sub-path (digits of levels base..d, then 0) of the next round's split that this
+              // charge belongs to; when it is a new one, this charge is its first node and what was
+              // counted before belongs to earlier sub-paths (a charge at a deeper level with digit
+              // 0 stays in the sub-path its shallower charge opened)
+              int j = 0;
+              for (int t = base; t < base + a.snap_ext; t++)
+                j = (j << 3) | (t < d ? uni(__builtin_amdgcn_readlane(st_k, t)) : (t == d ? k : 0));
+              if (j != (int)(snap_c >> 44)) {
+                snap_c = cur | ((long long)j << 44);
+                snap_n = total;
+              }
+            }
             total++;
             dn += (lane == d) ? 1 : 0;
             if (limited && ++cur > bud) { state = S_ABORT; continue; }
@@ -518,7 +540,8 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
           if (cm & (0xFFu << (kl + 1)) & 0xFFu) { total++; dn += (lane == l) ? 1 : 0; }
         }
         verdict = cap_stop ? 5 : 3;
-        A = B = 0;
+        A = cap_stop ? (uint64_t)snap_c : 0;
+        B = cap_stop ? (uint64_t)snap_n : 0;
       } else {
         verdict = 2;
         A = B = 0;

@@ -132,6 +132,13 @@ struct CompleteArgs {
   long long *nodes;
   uint64_t *outA, *outB;
   long long *depth_out;        // unsplit launches: [item][64] nodes counted at each depth (null: off)
+  // Resumed subtrees: items below ent_skip[e] were walked by the capped item whose subtree entry e
+  // is (the host carries their charges and nodes), so they end at once with nothing counted (null:
+  // none). snap_ext > 0: a capped item reports where it stopped for the next round's split of
+  // snap_ext levels: outA = charges before the first node of that split's sub-path it was in,
+  // plus the sub-path's index << 44; outB = the nodes counted before it.
+  const int *ent_skip;
+  int snap_ext;
 };
 
 }  // namespace wzfh

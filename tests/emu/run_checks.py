@@ -38,6 +38,11 @@ os.environ['WZFH_COMPLETE_EXACT_ITEM'] = '1'
 # walk abort items of up to 2,048 charges whole; a smaller cap or exact-item threshold multiplies
 # the emulated launches (cap 4: ~200 s on this emulation)
 plan = [(2, 0, c) for c in cases] + [(1, 64, c) for c in cases if (c['name'], c['budget']) in (('n13:5', 0), ('n13:23', 300))]
+# small caps: capped items inside the subtrees of capped items, each next round resuming from the
+# sub-path where the capped item stopped
+plan += [(2, 16, c) for c in cases if (c['name'], c['budget']) == ('n13:23', 0)]
+plan += [(2, 8, c) for c in cases if (c['name'], c['budget']) == ('n13:25', 0)]
+plan += [(2, 4, c) for c in cases if (c['name'], c['budget']) == ('n13:5', 0)]
 for split, cap, c in plan:
     os.environ['WZFH_COMPLETE_SPLIT'] = str(split)
     os.environ['WZFH_COMPLETE_ITEM_CAP'] = str(cap)
