@@ -217,7 +217,7 @@ def main():
     ap.add_argument('--load', type=float, default=0.6, help='Poisson rate as a fraction of the served rate')
     ap.add_argument('--max-batch', type=int, default=0, help='cells per stream launch (0: two slices)')
     ap.add_argument('--in-flight', type=int, default=8, help='served steps in flight at once')
-    ap.add_argument('--serve-caps', default='12,13,14,15',
+    ap.add_argument('--serve-caps', default='11,12,13,14',
                     help='completer charge caps (log2) tried on served steps after autotuning (empty: skip)')
     ap.add_argument('--serve-tune-steps', type=int, default=4, help='served steps per cap trial')
     ap.add_argument('--seed', type=int, default=44)
