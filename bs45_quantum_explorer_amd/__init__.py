@@ -39,7 +39,7 @@ class WzfhError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [(k, C.c_int) for k in ('n', 'a', 'b', 'c', 'd', 'm6', 'thm211b', 'thm212', 'canon', 'orbit_q',
                                         'endpos', 'order', 'rev', 'ab_prof', 'outer_k', 'no_canon_ab')] + \
-               [('abp_cap', C.c_longlong), ('orbit_qprune', C.c_int), ('reserved', C.c_int)]
+               [('abp_cap', C.c_longlong), ('orbit_qprune', C.c_int), ('a2', C.c_int)]
 
 
 class StreamOpts(C.Structure):

@@ -849,6 +849,21 @@ static int complete_launch(wzfh_problem &P, DeviceState *S, const uint64_t *cd, 
         if (i <= d || i >= P.L1 - 1 - d) r--;
       a.rcap[d][c] = (uint8_t)std::max(0, r);
     }
+  if (P.cfg.a2 == 2) {   // init_a2_tables (wz_match.cpp:1001-1016)
+    int blk_of[8];
+    for (int c = 0; c < 8; c++) blk_of[c] = -1;
+    a.a2 = 1;
+    for (int c = 0; c < P.m; c++) {
+      if (blk_of[c] >= 0) continue;
+      const int sr = ((P.L1 - 1 - c) % P.m + P.m) % P.m;
+      a.a2_r[a.a2_nblk] = (int8_t)c; a.a2_s[a.a2_nblk] = (int8_t)sr; a.a2_mid[a.a2_nblk] = 0;
+      blk_of[c] = blk_of[sr] = a.a2_nblk++;
+    }
+    const int half = P.L1 / 2;
+    if (P.L1 % 2 == 1) a.a2_mid[blk_of[half % P.m]] = 1;
+    for (int p = 0; p <= half && p < 33; p++)
+      for (int i = p; i < half; i++) a.a2_cnt[p][blk_of[i % P.m]]++;
+  }
   a.outer2 = S->outer2.as<uint8_t>(); a.outer3 = S->outer3.as<uint8_t>();
   a.pos_pack = 0;
   for (int sb = 0; sb < 16; sb++) a.pos_pack |= (uint64_t)(kOuter.pos_index[sb] + 1) << (4 * sb);
@@ -1390,6 +1405,7 @@ int problem_init(const wzfh_config &cfg_in, bool ab_side, bool with_device, wzfh
     return WZFH_EINVAL;
   }
   if (cfg->outer_k != 0 && cfg->outer_k != 2 && cfg->outer_k != 3) { set_error("outer_k must be 0, 2 or 3"); return WZFH_EINVAL; }
+  if (cfg->a2 != 0 && cfg->a2 != 2) { set_error("a2 must be 0 (off) or 2 (prune); the shadow mode 1 is not provided"); return WZFH_EINVAL; }
   wzfh_problem *P = new wzfh_problem();
   P->cfg = *cfg;
   P->ab_side = ab_side;

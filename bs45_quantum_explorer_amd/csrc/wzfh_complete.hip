@@ -98,6 +98,42 @@ __device__ __forceinline__ int quad_delta(const Partners &p, int a1, int b1, int
   return a1 * p.U1 + b1 * p.V1 + a2 * p.U2 + b2 * p.V2 + p.x * (a1 * a2 + b1 * b2);
 }
 
+// a2_row_ok (wz_match.cpp:1017-1036): residual (u, v) of a self-reflected block with q quads
+// left, each adding (+-2, +-2) or (0, 0)
+__device__ __forceinline__ bool a2_self_ok(int u, int v, int q) {
+  if ((u | v) & 1) return false;
+  if (abs(u) > 2 * q || abs(v) > 2 * q) return false;
+  return ((u - v) & 3) == 0;
+}
+
+// a2_row_ok: can the remaining positive quads of every mirror block reach the row (k, r words)
+// from the partial sums (PA, PB words, 64-biased bytes) at depth p? A quad of a block of two
+// residues adds a +-column of the 4x4 Hadamard matrix H, so with t = H (row - partial) the
+// block's q quads reach it iff t = 0 mod 4, |t|_1 / 4 <= q and q - |t|_1 / 4 is even.
+__device__ __forceinline__ bool a2_row_ok(const CompleteArgs &a, uint64_t kw, uint64_t rw, uint64_t PA,
+                                          uint64_t PB, int p) {
+  auto f = [](uint64_t w, int c) { return (int)(int8_t)((w >> (8 * c)) & 255u); };
+  auto pf = [](uint64_t w, int c) { return (int)((w >> (8 * c)) & 255u) - 64; };
+  for (int b = 0; b < a.a2_nblk; b++) {
+    const int r = a.a2_r[b], sr = a.a2_s[b], q = a.a2_cnt[p][b];
+    const int ur = f(kw, r) - pf(PA, r), vr = f(rw, r) - pf(PB, r);
+    if (r != sr) {
+      const int us = f(kw, sr) - pf(PA, sr), vs = f(rw, sr) - pf(PB, sr);
+      const int t0 = ur + vr + us + vs, t1 = ur - vr + us - vs, t2 = ur + vr - us - vs, t3 = ur - vr - us + vs;
+      if ((t0 | t1 | t2 | t3) & 3) return false;
+      const int l1 = (abs(t0) + abs(t1) + abs(t2) + abs(t3)) >> 2;
+      if (l1 > q || ((q - l1) & 1)) return false;
+    } else if (a.a2_mid[b]) {
+      if (!a2_self_ok(ur - 1, vr - 1, q) && !a2_self_ok(ur - 1, vr + 1, q) && !a2_self_ok(ur + 1, vr - 1, q) &&
+          !a2_self_ok(ur + 1, vr + 1, q))
+        return false;
+    } else if (!a2_self_ok(ur, vr, q)) {
+      return false;
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 // One wavefront per block. kLds: the row bitsets live in dynamic LDS; otherwise in the
@@ -452,6 +488,7 @@ __global__ __launch_bounds__(64) void complete_kernel(CompleteArgs a, const uint
                   const int k1 = (int)(int8_t)((kw >> (8 * c1)) & 255), k2 = (int)(int8_t)((kw >> (8 * c2)) & 255);
                   const int q1 = (int)(int8_t)((rw >> (8 * c1)) & 255), q2 = (int)(int8_t)((rw >> (8 * c2)) & 255);
                   ok = abs(k1 - p1a) <= r1 && abs(q1 - p1b) <= r1 && abs(k2 - p2a) <= r2 && abs(q2 - p2b) <= r2;
+                  if (ok && a.a2) ok = a2_row_ok(a, kw, rw, nPA, nPB, d + 1);
                 }
                 const uint64_t nb = __ballot(ok);
                 if (t0 + u < nw) {

@@ -139,6 +139,13 @@ struct CompleteArgs {
   // plus the sub-path's index << 44; outB = the nodes counted before it.
   const int *ent_skip;
   int snap_ext;
+  // Exact profile-row reachability (WZ_FH_A2=2, wz_match.cpp:977-1078): a2 != 0 also drops the
+  // rows of the filter whose residuals the remaining positive quads of some mirror block cannot
+  // reach. Block b covers residues a2_r[b], a2_s[b] (equal: self-reflected; a2_mid[b]: it holds
+  // the free odd-L middle); a2_cnt[p][b] = quads of block b at depths p..half-1.
+  int a2, a2_nblk;
+  int8_t a2_r[8], a2_s[8], a2_mid[8];
+  uint8_t a2_cnt[33][8];
 };
 
 }  // namespace wzfh

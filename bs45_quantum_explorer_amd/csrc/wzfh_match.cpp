@@ -215,9 +215,10 @@ int main(int argc, char **argv) {
   cfg.abp_cap = env_ll("WZ_FH_AB_PROF_CAP", 20000);
   if (const char *e = getenv("WZ_FH_AB_PROF")) cfg.ab_prof = atoi(e) != 0;
   cfg.orbit_qprune = env_ll("WZ_FH_ORBIT_QPRUNE", 0) != 0;
-  if (env_ll("WZ_FH_CD_PRUNE", 0) || env_ll("WZ_FH_A2", 0) || env_ll("WZ_FH_WALL_SEC", 0) ||
+  cfg.a2 = (int)env_ll("WZ_FH_A2", 0);   // wz_match.cpp:1370: 2 prunes profile rows (A2)
+  if (env_ll("WZ_FH_CD_PRUNE", 0) || (cfg.a2 != 0 && cfg.a2 != 2) || env_ll("WZ_FH_WALL_SEC", 0) ||
       env_set("WZ_FH_TARGET_C")) {
-    printf("RESULT: UNSUPPORTED FLAG (CD_PRUNE, A2, WALL_SEC, TARGET are not implemented)\n");
+    printf("RESULT: UNSUPPORTED FLAG (CD_PRUNE, A2 shadow mode, WALL_SEC, TARGET are not implemented)\n");
     return 2;
   }
   Telem TM;
@@ -289,8 +290,12 @@ int main(int argc, char **argv) {
     auto t0 = Clock::now();
     int r2 = wzfh_complete(P, cd, nullptr, 1, budget, &out);
     if (r2 != WZFH_OK && r2 != WZFH_EINTERNAL) { printf("RESULT: DEVICE ERROR (%s)\n", wzfh_last_error()); return 2; }
-    printf("COMPLETE_ONLY r=%d nodes=%lld budget=%lld outer_k=%d a2=0 sec=%g\n", out.verdict, out.nodes, budget,
-           cfg.outer_k, std::chrono::duration<double>(Clock::now() - t0).count());
+    printf("COMPLETE_ONLY r=%d nodes=%lld budget=%lld outer_k=%d a2=%d sec=%g\n", out.verdict, out.nodes, budget,
+           cfg.outer_k, cfg.a2, std::chrono::duration<double>(Clock::now() - t0).count());
+    // a2_print (wz_match.cpp:1039-1047): complete-only runs without rows, so A2 tests none
+    if (cfg.a2)
+      printf("A2SHADOW mode=%d rows_tested=0 rows_rejected=0 cuts=0 saved_nodes=0 total_nodes=%lld hit_under_cut=0 "
+             "cuts_by_depth= saved_by_depth=\n", cfg.a2, out.nodes);
     if (out.verdict == 0) {
       printf("*** BS(%d,%d) FOUND ***  (complete-only)\n", n + 1, n);
       print_seq("A", unpack(out.A, n + 1)); print_seq("B", unpack(out.B, n + 1));

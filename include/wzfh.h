@@ -68,7 +68,9 @@ typedef struct {
   long long abp_cap; /* WZ_FH_AB_PROF_CAP (default 20000)                           */
   int orbit_qprune;  /* WZ_FH_ORBIT_QPRUNE: Q-closure orbit elimination (needs canon,
                         the quad switch Q active and mod-6 cells; else ignored)        */
-  int reserved;
+  int a2;            /* WZ_FH_A2: 0 = off, 2 = exact profile-row reachability prune of the
+                        completer's row filter (wz_match.cpp:977-1078; needs rows); the
+                        shadow mode 1 is rejected with WZFH_EINVAL                       */
 } wzfh_config;
 
 typedef struct wzfh_problem wzfh_problem;

@@ -203,7 +203,29 @@ struct Problem {
   std::vector<std::vector<Row>> rowlists;
   bool ab_prof;
   double hcos[201][64], hsin[201][64];
+  // A2 (wz_match.cpp:991-1016): mirror blocks of residues and quads left per block and depth
+  int a2_nblk = 0, a2_r[8] = {0}, a2_s[8] = {0}, a2_mid[8] = {0}, a2_cnt[40][8] = {{0}};
+  mutable long long a2_tested = 0, a2_rejected = 0;
 };
+
+// init_a2_tables (wz_match.cpp:1001-1016): residue c and its reflection (L-1-c) mod m form a
+// block; an odd L's middle residue is self-reflected and free until the leaf.
+void init_a2(Problem &P) {
+  int L = P.L1, m = P.m, half = L / 2, blk_of[8];
+  for (int c = 0; c < 8; c++) blk_of[c] = -1;
+  P.a2_nblk = 0;
+  for (int c = 0; c < m; c++) {
+    if (blk_of[c] >= 0) continue;
+    int s = ((L - 1 - c) % m + m) % m;
+    P.a2_r[P.a2_nblk] = c; P.a2_s[P.a2_nblk] = s; P.a2_mid[P.a2_nblk] = 0;
+    blk_of[c] = blk_of[s] = P.a2_nblk++;
+  }
+  if (L % 2 == 1) P.a2_mid[blk_of[half % m]] = 1;
+  for (int p = 0; p <= half + 1 && p < 40; p++) {
+    for (int b = 0; b < 8; b++) P.a2_cnt[p][b] = 0;
+    for (int i = p; i < half; i++) P.a2_cnt[p][blk_of[i % m]]++;
+  }
+}
 
 std::string cell_key(const Vec &px, const Vec &py) {   // wz_match.cpp:1665-1671
   std::string k;
@@ -575,7 +597,31 @@ struct Completer {
     if (k == 2) return OT.t2[pat][R[0] / 2 + 2][R[1] / 2 + 4];
     return OT.t3[pat][R[0] / 2 + 2][R[1] / 2 + 4][R[2] / 2 + 6];
   }
-  bool row_filter(int d) {   // fh_abp_filter (wz_match.cpp:1055-1078), A2 off
+  static bool a2_self_ok(int u, int v, int q) {   // wz_match.cpp:1017-1021
+    if ((u | v) & 1) return false;
+    if (std::abs(u) > 2 * q || std::abs(v) > 2 * q) return false;
+    return ((u - v) & 3) == 0;
+  }
+  bool a2_row_ok(const Row &row, int p) const {   // wz_match.cpp:1022-1038
+    for (int b = 0; b < P.a2_nblk; b++) {
+      int r = P.a2_r[b], s = P.a2_s[b], q = P.a2_cnt[p][b];
+      int ur = row.k[r] - PA[r], vr = row.r[r] - PB[r];
+      if (r != s) {
+        int us = row.k[s] - PA[s], vs = row.r[s] - PB[s];
+        int t0 = ur + vr + us + vs, t1 = ur - vr + us - vs, t2 = ur + vr - us - vs, t3 = ur - vr - us + vs;
+        if ((t0 | t1 | t2 | t3) & 3) return false;
+        int l1 = (std::abs(t0) + std::abs(t1) + std::abs(t2) + std::abs(t3)) >> 2;
+        if (l1 > q || ((q - l1) & 1)) return false;
+      } else if (P.a2_mid[b]) {
+        if (!a2_self_ok(ur - 1, vr - 1, q) && !a2_self_ok(ur - 1, vr + 1, q) &&
+            !a2_self_ok(ur + 1, vr - 1, q) && !a2_self_ok(ur + 1, vr + 1, q)) return false;
+      } else if (!a2_self_ok(ur, vr, q)) {
+        return false;
+      }
+    }
+    return true;
+  }
+  bool row_filter(int d) {   // fh_abp_filter (wz_match.cpp:1055-1078), A2 after the capacity box
     auto &src = stack[d];
     auto &dst = stack[d + 1];
     dst.clear();
@@ -585,6 +631,10 @@ struct Completer {
       for (int c = 0; c < m; c++) {
         int rem = TIC[c] - PL[c], dk = row.k[c] - PA[c], dr = row.r[c] - PB[c];
         if (dk < -rem || dk > rem || dr < -rem || dr > rem) { ok = false; break; }
+      }
+      if (ok && P.cfg.a2) {
+        P.a2_tested++;
+        if (!a2_row_ok(row, d + 1)) { P.a2_rejected++; if (P.cfg.a2 == 2) ok = false; }
       }
       if (ok) dst.push_back(idx);
     }
@@ -754,6 +804,7 @@ void *wzo_create(const wzo_config *cfg) {
   P->pinC = (cfg->c == 0) && !cfg->canon;
   P->pinD = (cfg->d == 0) && !cfg->canon;
   P->ab_prof = cfg->ab_prof < 0 ? (cfg->m6 != 0) : (cfg->ab_prof != 0);
+  init_a2(*P);
   for (int j = 1; j <= 200; j++) {   // init_hall_tables: wz_match.cpp:106-114
     double th = j * M_PI / 100.0;
     for (int i = 0; i < P->L1 && i < 64; i++) { P->hcos[j][i] = cos(i * th); P->hsin[j][i] = sin(i * th); }
@@ -827,6 +878,12 @@ int wzo_complete(void *h, int pi, const int8_t *C, const int8_t *D, long long bu
   int r = K.run(C, D, A, B);
   if (nodes) *nodes = K.total;
   return r;
+}
+
+void wzo_a2_counts(void *h, long long *tested, long long *rejected) {
+  Problem &P = *(Problem *)h;
+  *tested = P.a2_tested; *rejected = P.a2_rejected;
+  P.a2_tested = P.a2_rejected = 0;
 }
 
 long long wzo_flat_score(int n, const int8_t *C, const int8_t *D) {   // wz_match.cpp:2569-2583

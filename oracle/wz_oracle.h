@@ -17,6 +17,8 @@
  *                                          budget aborts) + firsthit_runs.json
  *   - npaf / flat score                  : closed form + 43 banked solutions
  *   - outer (A1) tables                  : brute-force enumeration in tests
+ *   - A2 row reachability (a2)           : a2_runs.json (arm runs with WZ_FH_A2=2:
+ *                                          CAND lines, nodes, A2SHADOW row counts)
  *   - solution set (wzo_solutions)       : brute force over every quadruple at
  *                                          n = 6..9 in tests, the banked solutions
  */
@@ -41,6 +43,7 @@ typedef struct {
   int no_canon_ab;     /* WZ_FH_NO_CANON (completer root/reversal canon off) */
   long long abp_cap;   /* WZ_FH_AB_PROF_CAP (default 20000)                  */
   int no_cells;        /* 1: skip the cell list (completion-only use)        */
+  int a2;              /* WZ_FH_A2: 0 off, 1 count rows only, 2 prune rows    */
 } wzo_config;
 
 typedef struct {
@@ -67,6 +70,9 @@ long long wzo_stream_cell(void *h, int pi, long long cap, int8_t *out, long long
 int wzo_complete(void *h, int pi, const int8_t *C, const int8_t *D, long long budget,
                  long long *nodes, int8_t *A, int8_t *B);
 long long wzo_flat_score(int n, const int8_t *C, const int8_t *D);
+/* The A2 row counters summed over this problem's completions since the last call (the
+ * rows_tested / rows_rejected of the reference's A2SHADOW line), then cleared. */
+void wzo_a2_counts(void *h, long long *tested, long long *rejected);
 /* Summed NPAF at shift s (wz_match.cpp:184-192). */
 int wzo_npaf_at(const int8_t *A, const int8_t *B, int n1, const int8_t *C, const int8_t *D,
                 int n2, int s);

@@ -70,6 +70,24 @@ for batch in (cases[:2], cases[:1]):
         report(ok, 'complete_noalloc', len(batch), 'budget', budget, [(o.verdict, o.nodes) for o in got])
 P.close()
 
+# the profile-row filter with the A2 reachability prune (WZ_FH_A2=2) against the oracle: in-cell
+# completions under their cells' rows, split 2, of stream candidates whose node counts the prune
+# changes (n = 13: cell 3, candidates 24-31, exact and budget 112; n = 12: cell 43, hits)
+from oracle_lib import OracleProblem  # noqa: E402
+os.environ['WZFH_COMPLETE_SPLIT'] = '2'
+for n, sig, pi, j0, budgets in ((13, (2, 4, 3, 5), 3, 24, (0, 112)), (12, (1, 7, 0, 0), 43, 0, (0,))):
+    P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1, a2=2)
+    O = OracleProblem(n, sig, canon=1, order=1, a2=2)
+    _, cands, _ = O.stream(pi, j0 + 8, j0 + 8)
+    cds = [[wz.pack([int(v) for v in c[0]]), wz.pack([int(v) for v in c[1]])] for c in cands[j0:]]
+    for budget in budgets:
+        got = P.complete(np.array(cds, dtype=np.uint64), budget=budget, cell_of=[pi] * len(cds))
+        want = [tuple(O.complete(wz.unpack(cd[0], n), wz.unpack(cd[1], n), budget, pi=pi)[:2]) for cd in cds]
+        res = [(o.verdict, o.nodes) for o in got]
+        report(res == want, 'complete rows a2 n', n, 'cell', pi, 'budget', budget, res)
+    P.close()
+    O.close()
+
 # stream: per-cell counts and stream hash of small-n runs, with donation forced every 2 moves
 os.environ['WZFH_DONATE_MOVES'] = '2'
 os.environ['WZFH_DONATE_LOW'] = '1000000'

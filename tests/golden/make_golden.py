@@ -30,6 +30,9 @@ Fixtures:
                       stale CFGSIG, n = 19 front-only interrupts)
   telemetry.json      FH_TELEM v2 records (deterministic fields: counts, node sums, depth and rank
                       histograms) of the reference's telemetry gate cases at strides 1 and 64
+  a2_runs.json        full WZ_FIRSTHIT runs as firsthit_runs.json with WZ_FH_A2=2 (the exact
+                      profile-row reachability prune), with the arm's A2SHADOW row counts
+                      (n = 6..13 every class, n = 19 every class)
   qprune_cells.json   the Q-closure prune (WZ_FH_ORBIT_QPRUNE with the 64-element orbit
                       group): [qprune] counts, kept orbits, order-3 digests and the per-cell
                       streams of the cells left (n = 10..14 safe classes, n = 11, 13 ENDPOS)
@@ -224,10 +227,12 @@ CANDLINE = re.compile(r'^CAND idx=(\d+) ci=(-?\d+) pi=(\d+) r=(\d) nodes=(\d+)',
 FHLINE = re.compile(r'^FIRSTHIT: idx=(\d+) profile_rank=(\d+) nodes_this_cand=(\d+) score=(\d+)', re.M)
 
 
-def firsthit_case(n, sig, extra):
+def firsthit_case(n, sig, extra, outs=None):
     cfg = {**dict(WZ_FIRSTHIT=1, WZ_FH_M6=1, WZ_THM211B=1, WZ_THM212=1, WZ_FH_PROF_ORDER=1,
                   WZ_FH_ORBIT_CANON=1, WZ_FH_PROG_SEC=99999, WZ_FH_CAND_LOG=1), **extra}
     rc, out = run(n, sig, cfg)
+    if outs is not None:
+        outs.append(out)
     rec = dict(n=n, sig=list(sig), settings={k: v for k, v in cfg.items() if k != 'WZ_FH_PROG_SEC'},
                rc=rc, summary=summary(out),
                cand=[[int(g) for g in m.groups()] for m in CANDLINE.finditer(out)])
@@ -256,6 +261,37 @@ def gen_firsthit(jobs):
     (HERE / 'firsthit_runs.json').write_text(json.dumps(recs, separators=(',', ':')))
     print(f'firsthit_runs.json: {len(recs)} runs, {sum(len(r["cand"]) for r in recs)} completions, '
           f'{sum(1 for r in recs if "firsthit" in r)} hits')
+
+
+A2LINE = re.compile(r'^A2SHADOW mode=(\d+) rows_tested=(\d+) rows_rejected=(\d+) cuts=(\d+) saved_nodes=(\d+) '
+                    r'total_nodes=(\d+) hit_under_cut=(\d+)', re.M)
+
+
+def a2_case(n, sig, extra):
+    outs = []
+    rec = firsthit_case(n, sig, {**extra, 'WZ_FH_A2': 2}, outs)
+    m = A2LINE.findall(outs[0])
+    assert len(m) == 1, out[-400:]
+    rec['a2shadow'] = [int(g) for g in m[0]]
+    return rec
+
+
+def gen_a2(jobs):
+    tasks = []
+    for n in range(6, 14):
+        for sig in classes(n):
+            for budget in (0, 30):
+                tasks.append((n, sig, {'WZ_FH_AB_BUDGET': budget}))
+            tasks.append((n, sig, {'WZ_FH_AB_BUDGET': 40, 'WZ_FH_BUF_CAP': 8, 'WZ_FH_DRAIN_TOP': 3,
+                                   'WZ_FH_DRAIN_BATCHES': 2}))
+    for sig in classes(19):
+        tasks.append((19, sig, {'WZ_FH_AB_BUDGET': 500, 'WZ_FH_BUF_CAP': 64, 'WZ_FH_DRAIN_TOP': 8,
+                                'WZ_FH_MAX_CAND': 20000}))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        recs = list(ex.map(lambda t: a2_case(*t), tasks))
+    (HERE / 'a2_runs.json').write_text(json.dumps(recs, separators=(',', ':')))
+    print(f'a2_runs.json: {len(recs)} runs, {sum(len(r["cand"]) for r in recs)} completions, '
+          f'{sum(1 for r in recs if "firsthit" in r)} hits, {sum(r["a2shadow"][2] for r in recs)} rows rejected')
 
 
 def gen_n29(jobs):
@@ -564,7 +600,7 @@ def main():
         sys.exit(f'{REF} missing: run make -C oracle ref (needs the reference tree)')
     gens = dict(stream=gen_stream, complete=gen_complete, firsthit=gen_firsthit, n29=gen_n29,
                 solutions=gen_solutions, join=gen_join, join_signed=gen_join_signed, resume=gen_resume,
-                qprune=gen_qprune, telem=gen_telem)
+                qprune=gen_qprune, telem=gen_telem, a2=gen_a2)
     for name, g in gens.items():
         if not a.only or name in a.only.split(','):
             g(a.jobs)

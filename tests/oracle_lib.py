@@ -14,7 +14,7 @@ LIB = REPO / 'oracle' / '_build' / 'liboracle.so'
 class Config(C.Structure):
     _fields_ = [(k, C.c_int) for k in ('n', 'a', 'b', 'c', 'd', 'm6', 'thm211b', 'thm212', 'canon',
                                         'orbit_q', 'endpos', 'order', 'rev', 'ab_prof', 'outer_k',
-                                        'no_canon_ab')] + [('abp_cap', C.c_longlong), ('no_cells', C.c_int)]
+                                        'no_canon_ab')] + [('abp_cap', C.c_longlong), ('no_cells', C.c_int), ('a2', C.c_int)]
 
 
 class StreamStats(C.Structure):
@@ -43,6 +43,7 @@ def lib():
         L.wzo_create.restype = C.c_void_p
         L.wzo_create.argtypes = [C.POINTER(Config)]
         L.wzo_destroy.argtypes = [C.c_void_p]
+        L.wzo_a2_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.wzo_create_with_cells.restype = C.c_void_p
         L.wzo_create_with_cells.argtypes = [C.POINTER(Config), C.c_void_p, C.c_int]
         L.wzo_num_cells.argtypes = [C.c_void_p]
@@ -75,11 +76,11 @@ def _i8(a):
 
 class OracleProblem:
     def __init__(self, n, sig, m6=1, thm211b=1, thm212=1, canon=0, orbit_q=0, endpos=0, order=1,
-                 rev=0, ab_prof=-1, outer_k=3, no_canon_ab=0, abp_cap=20000, no_cells=0, cells=None):
+                 rev=0, ab_prof=-1, outer_k=3, no_canon_ab=0, abp_cap=20000, no_cells=0, cells=None, a2=0):
         self.n = n
         self.m = 6 if m6 else 3
         cfg = Config(n, *sig, m6, thm211b, thm212, canon, orbit_q, endpos, order, rev, ab_prof,
-                     outer_k, no_canon_ab, abp_cap, no_cells)
+                     outer_k, no_canon_ab, abp_cap, no_cells, a2)
         if cells is None:
             self.h = lib().wzo_create(C.byref(cfg))
         else:   # explicit (px, py) list: cell i of this problem is cells[i]
@@ -96,6 +97,12 @@ class OracleProblem:
 
     def __del__(self):
         self.close()
+
+    def a2_counts(self):
+        """(rows_tested, rows_rejected) of the A2 row test since the last call."""
+        t, r = C.c_longlong(), C.c_longlong()
+        lib().wzo_a2_counts(self.h, C.byref(t), C.byref(r))
+        return t.value, r.value
 
     def num_cells(self):
         return lib().wzo_num_cells(self.h)

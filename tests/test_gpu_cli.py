@@ -56,9 +56,24 @@ def summary(out):
 RUNS = firsthit_runs()
 
 
+A2_RUNS = load('a2_runs.json')
+
+
 @pytest.mark.parametrize('n', sorted(set(r['n'] for r in RUNS)))
 def test_cli_firsthit_runs_match_reference(n):
-    for rec in (r for r in RUNS if r['n'] == n):
+    _check_runs([r for r in RUNS if r['n'] == n])
+
+
+@pytest.mark.parametrize('n', sorted(set(r['n'] for r in A2_RUNS)))
+def test_cli_a2_runs_match_reference(n):
+    """WZ_FH_A2=2: the arm with the exact profile-row reachability prune (CAND lines, node
+    counts, first hit and summary of the reference's pruned arm runs)."""
+    _check_runs([r for r in A2_RUNS if r['n'] == n])
+
+
+def _check_runs(recs):
+    for rec in recs:
+        n = rec['n']
         rc, out = run_cli(n, rec['sig'], rec['settings'])
         assert rc == rec['rc'], (rec['settings'], out[-500:])
         got = [[int(g) for g in m.groups()] for m in CAND.finditer(out)]

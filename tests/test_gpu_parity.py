@@ -141,14 +141,16 @@ def test_complete_split_abort_search_matches_reference(key, split, cap, monkeypa
     _complete_group([c for c in GROUPS[key] if c['budget'] > 0])
 
 
+@pytest.mark.parametrize('a2', [0, 2])
 @pytest.mark.parametrize('n,sig,budgets', [(13, (2, 4, 3, 5), (0, 30)), (19, (0, 2, 5, 7), (0, 500)),
-                                           (12, (1, 7, 0, 0), (0, 7))])
-def test_in_cell_completion_with_profile_rows_matches_oracle(n, sig, budgets):
+                                           (12, (1, 7, 0, 0), (0, 7)), (29, (0, 6, 1, 9), (2000, 20000))])
+def test_in_cell_completion_with_profile_rows_matches_oracle(n, sig, budgets, a2):
     """The in-stream completer (cell's A,B profile rows applied, wz_match.cpp:2505-2530) against
-    the oracle, which is pinned to the reference's full arm runs (firsthit_runs.json)."""
+    the oracle, which is pinned to the reference's full arm runs (firsthit_runs.json; with the
+    A2 row prune, WZ_FH_A2=2: a2_runs.json)."""
     from oracle_lib import OracleProblem
-    P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1)
-    O = OracleProblem(n, sig, canon=1, order=1)
+    P = wz.Problem(n, sig, m6=1, thm211b=1, thm212=1, canon=1, order=1, a2=a2)
+    O = OracleProblem(n, sig, canon=1, order=1, a2=a2)
     try:
         cells = P.live_cells()[:60]
         st, offs, cd = P.stream(cells, cand_cap=12)

@@ -10,14 +10,19 @@ line (idx, cell, nodes, score), the printed A,B,C,D and the summary counters mus
 equal. Then, for every cell the arm drained as a single sorted batch, wzo_cell_search (the
 unit the GPU search is checked against) must give the same per-rank verdicts, nodes and
 scores, the same streamed count and the same hit.
+
+The same replay with WZ_FH_A2=2 (a2_runs.json) pins the oracle's exact profile-row reachability
+prune: the CAND lines and node counts of the pruned arm, and the arm's A2SHADOW row counts
+(rows tested and rejected by the A2 test, summed over the arm's completions).
 """
 import numpy as np
 import pytest
 
-from golden_data import firsthit_runs
+from golden_data import firsthit_runs, load
 from oracle_lib import OracleProblem, flat_score
 
 RUNS = firsthit_runs()
+A2_RUNS = load('a2_runs.json')
 
 
 def _stream_all(P, pi, cap):
@@ -30,7 +35,8 @@ def _stream_all(P, pi, cap):
 def replay(run, check_cell_search=True):
     n, sig, s = run['n'], run['sig'], run['settings']
     P = OracleProblem(n, sig, canon=s.get('WZ_FH_ORBIT_CANON', 0), order=s.get('WZ_FH_PROF_ORDER', 1),
-                      rev=s.get('WZ_FH_STREAM_REV', 0))
+                      rev=s.get('WZ_FH_STREAM_REV', 0), a2=s.get('WZ_FH_A2', 0))
+    a2_rows = [0, 0]
     buf_cap = s.get('WZ_FH_BUF_CAP', 500000)
     top = s.get('WZ_FH_DRAIN_TOP', 0)
     nbat = max(1, s.get('WZ_FH_DRAIN_BATCHES', 1))
@@ -85,6 +91,9 @@ def replay(run, check_cell_search=True):
                 break
         lines += cell_lines
         cand += take
+        t, r = P.a2_counts()
+        a2_rows[0] += t
+        a2_rows[1] += r
         single_batch = nb <= 1 and not stopped
         if check_cell_search and single_batch and (top > 0 or take <= buf_cap):
             # the same cell through wzo_cell_search: one sorted batch of the first buf_cap
@@ -99,11 +108,13 @@ def replay(run, check_cell_search=True):
                 assert list(res.A[:n + 1]) == hit['A'] and list(res.B[:n + 1]) == hit['B']
                 assert list(res.C[:n]) == hit['C'] and list(res.D[:n]) == hit['D']
             assert res.ab_nodes == sum(x[4] for x in cell_lines)
+            P.a2_counts()   # the cell search's own row tests are not the arm's
             checked += 1
         if hit or stopped:
             break
     P.close()
     tot['candidates_streamed'] = cand
+    replay.a2_rows = tuple(a2_rows)
     return lines, hit, tot, checked
 
 
@@ -119,6 +130,41 @@ def test_oracle_replays_reference_arm_run(i):
         assert v == run['summary'][k], k
     if run['settings'].get('WZ_FH_DRAIN_BATCHES', 1) == 1:
         assert checked > 0   # at least the hit cell went through wzo_cell_search
+
+
+@pytest.mark.parametrize('i', range(len(A2_RUNS)))
+def test_oracle_replays_reference_a2_arm_run(i):
+    run = A2_RUNS[i]
+    lines, hit, tot, checked = replay(run)
+    assert lines == run['cand']
+    assert hit is not None and hit['firsthit'] == run['firsthit']
+    for k in 'ABCD':
+        assert hit[k] == run[k]
+    for k, v in tot.items():
+        assert v == run['summary'][k], k
+    mode, tested, rejected, cuts, saved, total, under = run['a2shadow']
+    assert (mode, cuts, saved, under) == (2, 0, 0, 0)
+    assert replay.a2_rows == (tested, rejected)
+    assert total == tot['total_AB_nodes']
+
+
+def test_a2_prune_cuts_nodes_and_keeps_hits():
+    """Against the unpruned arm runs of the same settings: the A2 prune removes rows whose
+    residuals no completion can reach, so an uncapped arm finds the same first hit with no more
+    completer nodes, and some runs need strictly fewer."""
+    base = {(r['n'], tuple(r['sig']), tuple(sorted(r['settings'].items()))): r for r in RUNS}
+    pairs, fewer = 0, 0
+    for run in A2_RUNS:
+        s = {k: v for k, v in run['settings'].items() if k != 'WZ_FH_A2'}
+        key = (run['n'], tuple(run['sig']), tuple(sorted(s.items())))
+        if key not in base or s.get('WZ_FH_AB_BUDGET') != 0:
+            continue
+        ref = base[key]
+        assert run['firsthit'][:2] == ref['firsthit'][:2] and run['A'] == ref['A'] and run['B'] == ref['B']
+        assert run['summary']['total_AB_nodes'] <= ref['summary']['total_AB_nodes']
+        fewer += run['summary']['total_AB_nodes'] < ref['summary']['total_AB_nodes']
+        pairs += 1
+    assert pairs >= 25 and fewer > 0
 
 
 def test_replay_detects_a_wrong_tie_order():
