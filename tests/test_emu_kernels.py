@@ -23,7 +23,7 @@ def emu_lib():
 
 def test_kernel_logic_on_cpu_emulation_matches_golden(emu_lib):
     r = subprocess.run([sys.executable, '-u', str(HERE / 'emu' / 'run_checks.py')], capture_output=True, text=True,
-                       timeout=540)
+                       timeout=900)
     out = r.stdout
     assert r.returncode == 0, out[-2000:] + r.stderr[-2000:]
     assert 'BAD' not in out, out
